@@ -1,0 +1,210 @@
+/*
+ * ts_fvm.h -- C ABI of the MI355X explicit finite-volume step library (libts_fvm.so).
+ *
+ * The library replaces one explicit step of TurboStructured's
+ * ExplicitRungeKuttaFVM<Reconstruction> (src/Methods/ExplicitRungeKuttaFVM.h:1440-1491):
+ * device ghost fill (src/kernel/kernel.h:693-915), reconstruction
+ * (ExplicitRungeKuttaFVM.h:879-925), per-face Riemann flux and flux divergence
+ * (ExplicitRungeKuttaFVM.h:1150-1424), the CFL time step (ExplicitRungeKuttaFVM.h:1427-1437)
+ * and the update U += dt*R/V (src/kernel/kernel.h:319-331).  All of it runs as hand-written
+ * HIP kernels for gfx950 on one GPU; the MPI halo exchange (kernel.h:334-690) is not needed
+ * because the whole domain lives in one GPU's HBM.
+ *
+ * Conventions (no exceptions cross this boundary):
+ *   - every entry point returns TS_OK (0) or a negative TS_ERR_* code;
+ *   - ts_last_error() returns the message of the last failure on that context;
+ *   - one context per host thread; a context owns one HIP stream and all device memory;
+ *   - state arrays that cross the boundary use the reference layout: array-of-structures,
+ *     5 doubles per cell (rho, rho*u, rho*v, rho*w, rho*E), ghost layers included,
+ *     serial index (k*nYAll + j)*nXAll + i (src/kernel/grid.h:64-67, kernel.h:93-96).
+ * Enumerations keep the numeric values of the reference's own enums so that a host can
+ * static_cast them (src/Methods/MethodConfiguration.h:5-23,
+ * src/BoundaryConditions/BoundaryConditionConfiguration.h:12-19).
+ */
+#ifndef TS_FVM_H
+#define TS_FVM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TS_ABI_VERSION 1
+#define TS_NVAR 5
+#define TS_MAX_BC 8
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+    TS_OK = 0,
+    TS_ERR_INVALID = -1,     /* bad argument or configuration */
+    TS_ERR_HIP = -2,         /* a HIP runtime call failed */
+    TS_ERR_UNPHYSICAL = -3,  /* NaN, non-positive density or pressure, or dt <= 0 */
+    TS_ERR_UNSUPPORTED = -4, /* configuration not built into this library */
+    TS_ERR_NOMEM = -5,       /* device memory exhausted or over the admission budget */
+    TS_ERR_BUSY = -6         /* request not finished yet / queue full */
+};
+
+/* ---- method enums (values of the reference's enum class members) ----------------------- */
+/* RPSolver (MethodConfiguration.h:12-16); HLLC is an extension with no reference counterpart */
+enum { TS_RIEMANN_GODUNOV = 0, TS_RIEMANN_ROE = 1, TS_RIEMANN_NONE = 2, TS_RIEMANN_HLLC = 3 };
+/* Reconstruction (MethodConfiguration.h:5-10) */
+enum { TS_RECON_PWC = 0, TS_RECON_LINEAR = 1, TS_RECON_ENO2 = 2, TS_RECON_LINEAR_LIM = 3 };
+/* LimiterType (MethodConfiguration.h:18-21) */
+enum { TS_LIMITER_BJ = 0, TS_LIMITER_VENKAT = 1 };
+/* BoundaryConditionType (BoundaryConditionConfiguration.h:12-19) */
+enum {
+    TS_BC_WALL = 0,
+    TS_BC_SYMMETRY = 1,
+    TS_BC_MOVING_WALL = 2,
+    TS_BC_SUBSONIC_INLET = 3,
+    TS_BC_SUBSONIC_OUTLET = 4,
+    TS_BC_NATURAL = 5
+};
+/* arithmetic of the residual */
+enum { TS_FP64 = 0, TS_FP32_RESIDUAL = 1 };
+/* sides, in the order the ghost fill visits them */
+enum { TS_XLEFT = 0, TS_XRIGHT = 1, TS_YLEFT = 2, TS_YRIGHT = 3, TS_ZLEFT = 4, TS_ZRIGHT = 5 };
+
+/* One boundary condition: mirrors BoundaryConditionConfiguration (BoundaryConditionConfiguration.h:22-36). */
+typedef struct ts_bc_config {
+    int type;           /* TS_BC_* */
+    double velocity[3]; /* MovingWall wall velocity / SubsonicInlet velocity */
+    double pstatic;     /* SubsonicInlet static pressure */
+    double density;     /* SubsonicInlet density */
+} ts_bc_config;
+
+/*
+ * Solver configuration: the fields of KernelConfiguration (src/kernel/KernelConfiguration.h:21-93)
+ * and MethodConfiguration (MethodConfiguration.h:26-37) that the inviscid step reads.
+ */
+typedef struct ts_config {
+    int ndims;           /* 1, 2 or 3 */
+    int nx, ny, nz;      /* interior cells per axis (ny = nz = 1 below 2-D / 3-D) */
+    int ghost;           /* DummyLayerSize; 1 (every reference scenario uses 1) */
+    double lx, ly, lz;   /* domain lengths; the domain starts at 0 on every axis */
+    int periodic[3];     /* isPeriodicX/Y/Z */
+    /* optional cell-centre coordinates and sizes per axis, ghost layers included
+       (n + 2*ghost entries, grid.h:45-50); NULL = uniform grid built as grid.h:164-218 does */
+    const double* coord[3];
+    const double* h[3];
+    double gamma;        /* Gamma */
+    double cfl;          /* CFL */
+    double eps;          /* Eps: Harten entropy-fix coefficient of the Roe solver */
+    int riemann;         /* TS_RIEMANN_* */
+    int recon;           /* TS_RECON_* */
+    int limiter;         /* TS_LIMITER_* (TS_RECON_LINEAR_LIM only) */
+    int rk_stages;       /* 1 = forward Euler (the reference's step), 3 = SSP-RK3 (extension) */
+    int precision;       /* TS_FP64 or TS_FP32_RESIDUAL */
+    /* boundary conditions: a table indexed by marker (MyConditions), a marker per side
+       (DomainBCinfo::SetMarker), and optional per-boundary-face marker arrays for sides that
+       carry several conditions (DomainBCinfo::getMarker evaluated at each face centre).
+       Face arrays are ordered (k*ny + j) for x sides, (k*nx + i) for y sides, (j*nx + i) for z. */
+    int n_bc;
+    ts_bc_config bc[TS_MAX_BC];
+    int side_marker[6];
+    const uint8_t* face_marker[6];
+    int device;          /* HIP device ordinal */
+} ts_config;
+
+/* StepInfo (kernel.h:56) plus the per-variable residual norms of UpdateSolution (kernel.h:326). */
+typedef struct ts_step_info {
+    double time;
+    double dt;
+    long long iteration;
+    double residual_norm[5];
+    int steps_done;      /* steps executed by this ts_step call */
+} ts_step_info;
+
+typedef struct ts_ctx ts_ctx;
+
+/* Create a solver context: validates the configuration, builds the grid, allocates the
+   device state.  Replaces Kernel::Init + InitializeMethod (kernel.h:178-281, ERK:39-75). */
+int ts_create(const ts_config* cfg, ts_ctx** out);
+int ts_destroy(ts_ctx* ctx);
+const char* ts_last_error(const ts_ctx* ctx);
+
+/* Number of doubles of the reference state array (nXAll*nYAll*nZAll*5, kernel.h:240). */
+size_t ts_state_size(const ts_ctx* ctx);
+
+/* Upload / download the conserved state in the reference AoS layout, ghosts included.
+   Ghost values are ignored on upload; on download they hold the boundary-condition images of
+   the current interior (kernel.h:693-915 applied to the returned state). */
+int ts_set_state(ts_ctx* ctx, const double* aos_values, size_t n_doubles);
+int ts_get_state(ts_ctx* ctx, double* aos_values, size_t n_doubles);
+
+/* Restore Time and Iteration after LoadSolution (kernel.h:1031-1079). */
+int ts_set_time(ts_ctx* ctx, double time, long long iteration);
+
+/*
+ * Run up to max_steps explicit steps (ERK:1440-1491 each), stopping after the step at which
+ * Time >= t_max (kernel.h:1841) or Time == next_snapshot_time.  dt is clamped so that
+ * Time + dt lands on next_snapshot_time when that is > 0 (ERK:1433).  Steps are issued
+ * without host synchronisation; the call returns after the last one has finished.
+ */
+int ts_step(ts_ctx* ctx, int max_steps, double t_max, double next_snapshot_time,
+            ts_step_info* out);
+
+/* Enable computation of the residual norms sum(V*|R|) of UpdateSolution (kernel.h:326) for
+   forward-Euler steps (off by default: they cost one extra reduction per step). */
+int ts_enable_residual_norms(ts_ctx* ctx, int enable);
+
+/* Device-side log of the dt of every step (for comparing dt sequences). */
+int ts_enable_dt_log(ts_ctx* ctx, long long capacity);
+int ts_get_dt_log(ts_ctx* ctx, double* out, long long capacity, long long* n_out);
+
+/* Per-kernel timing with HIP events recorded on the context's stream around every launch.
+   kind: 0 = dt pass, 1 = stage 1 / Euler stage, 2 = stage 2, 3 = stage 3, 4 = all. */
+int ts_enable_kernel_timing(ts_ctx* ctx, int enable);
+int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* launches);
+
+/* Device bytes held by the context and the algorithmic HBM bytes per cell of each kernel
+   kind (compulsory state traffic: one state array = 5*sizeof(real) per cell). */
+int ts_memory_footprint(const ts_ctx* ctx, size_t* device_bytes);
+int ts_algorithmic_bytes_per_cell(const ts_ctx* ctx, int kind, double* bytes);
+
+/* ---- batching and serving (extension: many independent problems on one GPU) ------------- */
+typedef struct ts_batch ts_batch;
+
+typedef struct ts_batch_config {
+    int device;
+    size_t hbm_budget_bytes; /* admission limit on the device memory of resident problems */
+    int max_active;          /* maximum number of problems advanced together */
+    int use_graphs;          /* capture the per-step launches of a fixed batch in a hipGraph */
+    int autotune;            /* pick tile shapes with the built-in autotuner */
+} ts_batch_config;
+
+int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out);
+int ts_batch_destroy(ts_batch* b);
+const char* ts_batch_last_error(const ts_batch* b);
+
+/* Queue one solve.  init_state is the reference AoS array of cfg's grid.  The solve runs until
+   max_steps steps are done or Time reaches t_end (the last dt is clamped to land on t_end).
+   Thread-safe with respect to ts_batch_run on another thread. */
+int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
+                    int max_steps, double t_end, long long* ticket);
+
+/* Admit queued solves that fit the budget, then advance every resident solve by up to n_steps
+   steps (continuous batching: solves join and leave between steps). Returns the number of
+   solves still resident or queued in *pending. */
+int ts_batch_run(ts_batch* b, int n_steps, int* pending);
+
+/* done = 1 when the ticket finished; then out_state (may be NULL) receives the final state in
+   the reference AoS layout and info its step information; the ticket is released. */
+int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state,
+                  ts_step_info* info);
+
+/* Serving: a background thread runs ts_batch_run continuously. */
+int ts_batch_start_server(ts_batch* b);
+int ts_batch_stop_server(ts_batch* b);
+
+/* Autotuner: time the candidate tile shapes for cfg's kernel family at cfg's size and store
+   the winner in the process-wide tuning table (also used by later ts_create calls). */
+int ts_autotune(const ts_config* cfg, int* chosen_rows_per_tile, double* best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TS_FVM_H */

@@ -1,0 +1,209 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle, FP64.
+
+Bar: dt of every step and the conserved interior state agree to 1e-12 relative (normwise per
+variable: max|gpu - oracle| <= 1e-12 * max|oracle|).  Not bit-exact: the GPU contracts
+multiply-adds and uses reciprocal-based division.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from turbostructred_amd import capi
+from oracle import pyoracle
+import util
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def run_pair(cfg, U, steps, t_max=1e300, snap=0.0, chunks=1):
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    gpu.enable_dt_log(steps + 8)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U)
+    dts_ref = []
+    per = max(1, steps // chunks)
+    done = 0
+    while done < steps:
+        k = min(per, steps - done)
+        ginfo = gpu.step(k, t_max, snap)
+        rinfo, rd = ref.step(k, t_max, snap, log=True)
+        dts_ref.extend(rd)
+        assert ginfo.steps_done == rinfo.steps_done, (ginfo.steps_done, rinfo.steps_done)
+        done += k
+        if ginfo.steps_done < k:
+            break
+    dts_gpu = gpu.dt_log(steps + 8)
+    a = capi.interior(cfg, gpu.get_state())
+    b = capi.interior(cfg, ref.get_state())
+    return a, b, np.array(dts_gpu), np.array(dts_ref), ginfo, rinfo
+
+
+def check(a, b, dg, dr, gi, ri, tol=TOL):
+    assert len(dg) == len(dr), (len(dg), len(dr))
+    np.testing.assert_array_less(np.abs(dg - dr), tol * np.abs(dr) + 1e-300)
+    assert abs(gi.time - ri.time) <= tol * abs(ri.time)
+    assert gi.iteration == ri.iteration
+    err = util.normwise_rel(a, b)
+    assert np.all(err <= tol), err
+    assert np.all(np.isfinite(a))
+
+
+def test_sod_reference_config():
+    """C1: Sod, 1000 cells, piecewise constant + Roe (Eps=0), forward Euler, CFL 0.4, Natural
+    BCs, run until Time >= 0.2 (the last step overshoots, kernel.h:1841)."""
+    cfg = capi.make_config(ndims=1, nx=1000, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC,
+                           bcs=[dict(type=capi.BC_NATURAL)])
+    U = util.sod(cfg)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 100000, t_max=0.2)
+    assert ri.time >= 0.2 and ri.steps_done > 2000
+    check(a, b, dg, dr, gi, ri)
+    # dt0 closed form: 0.4 * h / (2 sqrt(1.4))
+    assert abs(dg[0] - 0.4 * 1e-3 / (2 * np.sqrt(1.4))) <= 1e-15
+
+
+def smooth_1d(g):
+    """smooth wave plus a weak density jump: stays physical for every scheme, including the
+    unlimited central-linear one with forward Euler"""
+    def f(x, y, z):
+        rho = 1 + 0.3 * np.sin(2 * np.pi * x) + 0.2 * (x > 0.6)
+        return util.prim2cons(rho, 0.3 * np.cos(2 * np.pi * x), 0 * x, 0 * x, 1 + 0.1 * np.sin(4 * np.pi * x), g)
+    return f
+
+
+def smooth_2d(g):
+    def f(x, y, z):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) + 0.1 * (x + y > 0.9)
+        u = 0.2 * np.cos(2 * np.pi * y)
+        v = -0.15 * np.sin(2 * np.pi * x)
+        p = 1 + 0.1 * np.cos(2 * np.pi * (x - y))
+        return util.prim2cons(rho, u, v, 0 * x, p, g)
+    return f
+
+
+RECONS = [(capi.RECON_PWC, 0), (capi.RECON_LINEAR, 0), (capi.RECON_ENO2, 0),
+          (capi.RECON_LINEAR_LIM, capi.LIMITER_BJ), (capi.RECON_LINEAR_LIM, capi.LIMITER_VENKAT)]
+FLUXES = [capi.RIEMANN_ROE, capi.RIEMANN_HLLC, capi.RIEMANN_GODUNOV]
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_1d_matrix(recon, lim, flux, stages):
+    cfg = capi.make_config(ndims=1, nx=157, cfl=0.4, riemann=flux, recon=recon, limiter=lim,
+                           rk_stages=stages, eps=0.1 if flux == capi.RIEMANN_ROE else 0.0,
+                           bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL)],
+                           side_marker=(0, 1, 0, 0, 0, 0))
+    U = util.fill_state(cfg, smooth_1d(cfg.gamma))
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 60)
+    check(a, b, dg, dr, gi, ri)
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_2d_matrix(recon, lim, flux, stages):
+    cfg = capi.make_config(ndims=2, nx=131, ny=70, ly=0.8, cfl=0.45, riemann=flux, recon=recon, limiter=lim,
+                           rk_stages=stages,
+                           bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_SYMMETRY), dict(type=capi.BC_WALL)],
+                           side_marker=(0, 1, 0, 2, 0, 0))
+    if recon == capi.RECON_LINEAR:  # unlimited: smooth data only
+        U = util.fill_state(cfg, smooth_2d(cfg.gamma))
+    else:  # Lax-Liu configuration 3 with shifted interfaces
+        U = util.quadrants(cfg, xc=0.45, yc=0.37)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 25)
+    check(a, b, dg, dr, gi, ri)
+
+
+@pytest.mark.parametrize("periodic", [(1, 0, 0), (0, 1, 0), (1, 1, 0)])
+def test_2d_periodic(periodic):
+    cfg = capi.make_config(ndims=2, nx=64, ny=50, cfl=0.45, periodic=periodic, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3,
+                           bcs=[dict(type=capi.BC_NATURAL)])
+    g = cfg.gamma
+
+    def f(x, y, z):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * (x + y))
+        return util.prim2cons(rho, 1.0 + 0 * x, -0.5 + 0 * x, 0 * x, 1.0 + 0 * x, g)
+
+    U = util.fill_state(cfg, f)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 30)
+    check(a, b, dg, dr, gi, ri)
+
+
+def test_2d_boundary_conditions_all_types():
+    """Every BC type, including a composite side given by per-face markers."""
+    bcs = [dict(type=capi.BC_NATURAL), dict(type=capi.BC_SYMMETRY), dict(type=capi.BC_WALL),
+           dict(type=capi.BC_MOVING_WALL, velocity=(0.3, -0.1, 0.0)),
+           dict(type=capi.BC_SUBSONIC_INLET, velocity=(0.2, 0.1, 0.0), pstatic=0.9, density=1.1),
+           dict(type=capi.BC_SUBSONIC_OUTLET)]
+    nx, ny = 90, 66
+    xl = (np.arange(ny) % 3).astype(np.uint8)  # composite left side: Natural / Symmetry / Wall
+    cfg = capi.make_config(ndims=2, nx=nx, ny=ny, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=1, bcs=bcs, side_marker=(0, 5, 3, 4, 0, 0),
+                           face_marker=[xl, None, None, None, None, None])
+    U = util.fill_state(cfg, smooth_2d(cfg.gamma))
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 40)
+    check(a, b, dg, dr, gi, ri)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 61), (61, 3), (120, 1 + 16), (60, 33)])
+def test_2d_edge_sizes(shape):
+    nx, ny = shape
+    cfg = capi.make_config(ndims=2, nx=nx, ny=ny, cfl=0.4, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_ENO2,
+                           rk_stages=3, bcs=[dict(type=capi.BC_NATURAL)])
+    U = util.quadrants(cfg)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 10)
+    check(a, b, dg, dr, gi, ri)
+
+
+def test_snapshot_clamp_and_termination():
+    cfg = capi.make_config(ndims=1, nx=300, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC)
+    U = util.sod(cfg)
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U)
+    gi = gpu.step(10000, 1.0, 0.05)
+    ri = ref.step(10000, 1.0, 0.05)
+    assert gi.time == 0.05 and ri.time == 0.05  # the last dt lands on the snapshot time
+    assert gi.steps_done == ri.steps_done
+    gi = gpu.step(10000, 0.1, 0.1)
+    ri = ref.step(10000, 0.1, 0.1)
+    assert gi.time == ri.time == 0.1
+    err = util.normwise_rel(capi.interior(cfg, gpu.get_state()), capi.interior(cfg, ref.get_state()))
+    assert np.all(err <= TOL)
+
+
+def test_restart_continues_identically():
+    """ts_get_state -> new context -> ts_set_state/ts_set_time continues the same trajectory."""
+    cfg = capi.make_config(ndims=2, nx=70, ny=40, cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3)
+    U = util.quadrants(cfg)
+    s1 = capi.Solver(cfg)
+    s1.set_state(U)
+    s1.step(10)
+    mid = s1.get_state()
+    t_mid = s1.step(0).time
+    s1.step(10)
+    final1 = s1.get_state()
+    s2 = capi.Solver(cfg)
+    s2.set_state(mid)
+    s2.set_time(t_mid, 10)
+    s2.step(10)
+    final2 = s2.get_state()
+    np.testing.assert_array_equal(capi.interior(cfg, final1), capi.interior(cfg, final2))
+
+
+def test_unphysical_state_reports_error():
+    cfg = capi.make_config(ndims=1, nx=100, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC)
+    U = util.sod(cfg)
+    U = U.reshape(-1, 5)
+    U[40, 0] = np.nan
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    with pytest.raises(capi.TSError) as e:
+        gpu.step(5)
+    assert e.value.code == capi.TS_ERR_UNPHYSICAL

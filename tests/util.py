@@ -1,0 +1,81 @@
+"""Shared helpers for the tests: initial conditions in the reference AoS layout and error norms."""
+import numpy as np
+
+from turbostructred_amd import capi
+
+
+def prim2cons(rho, u, v, w, p, gamma):
+    rho, u, v, w, p = np.broadcast_arrays(*[np.asarray(x, dtype=np.float64) for x in (rho, u, v, w, p)])
+    return np.stack([rho, rho * u, rho * v, rho * w, p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)], axis=-1)
+
+
+def centres(cfg, axis):
+    """Cell-centre coordinates with ghosts, built like grid.h:164-218 (running sums)."""
+    n = [cfg.nx, cfg.ny, cfg.nz][axis]
+    L = [cfg.lx, cfg.ly, cfg.lz][axis]
+    if axis >= cfg.ndims:
+        return np.zeros(1)
+    stp = L / n
+    xs = []
+    xc = -stp + 0.5 * stp
+    xs.append(xc)
+    xc += stp
+    for _ in range(n):
+        xs.append(xc)
+        xc += 0.5 * stp * (1.0 + 1.0)
+    xs.append(xc)
+    return np.array(xs)
+
+
+def fill_state(cfg, fn):
+    """Point values at cell centres (Kernel::SetInitialConditions, kernel.h:125-146) for the
+    interior; ghosts left at the interior value next to them so every entry is physical."""
+    X = np.meshgrid(centres(cfg, 2), centres(cfg, 1), centres(cfg, 0), indexing="ij")
+    z, y, x = X
+    U = fn(x, y, z)
+    return np.ascontiguousarray(U.reshape(capi.state_shape(cfg)))
+
+
+def sod(cfg, x0=0.5, left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1), axis=0):
+    g = cfg.gamma
+
+    def f(x, y, z):
+        c = [x, y, z][axis]
+        L = c < x0
+        rho = np.where(L, left[0], right[0])
+        un = np.where(L, left[1], right[1])
+        p = np.where(L, left[2], right[2])
+        vel = [0 * x, 0 * x, 0 * x]
+        vel[axis] = un
+        return prim2cons(rho, vel[0], vel[1], vel[2], p, g)
+
+    return fill_state(cfg, f)
+
+
+LAX_LIU_3 = {  # (rho, u, v, p): NE, NW, SW, SE
+    "NE": (1.5, 0.0, 0.0, 1.5),
+    "NW": (0.5323, 1.206, 0.0, 0.3),
+    "SW": (0.138, 1.206, 1.206, 0.029),
+    "SE": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def quadrants(cfg, states=LAX_LIU_3, xc=0.5, yc=0.5):
+    g = cfg.gamma
+
+    def f(x, y, z):
+        out = np.zeros(x.shape + (5,))
+        for name, (r, u, v, p) in states.items():
+            m = ((x >= xc) if name[1] == "E" else (x < xc)) & ((y >= yc) if name[0] == "N" else (y < yc))
+            out[m] = prim2cons(r, u, v, 0.0, p, g)
+        return out
+
+    return fill_state(cfg, f)
+
+
+def normwise_rel(a, b):
+    """max |a-b| / max |b| per conserved variable over the interior."""
+    a = np.asarray(a).reshape(-1, 5)
+    b = np.asarray(b).reshape(-1, 5)
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return (np.abs(a - b).max(axis=0) / scale)

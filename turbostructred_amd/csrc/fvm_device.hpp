@@ -1,0 +1,561 @@
+// fvm_device.hpp -- device-side physics of the explicit finite-volume step (gfx950).
+//
+// Point-wise pieces used by the fused stage kernels: boundary images, cell reconstruction and
+// the Riemann fluxes, templated on the arithmetic type R (double, or float for the
+// FP32-residual form).  Each function cites the reference routine whose result it reproduces;
+// the arithmetic is written for CDNA (fused multiply-add, reciprocal instead of division where
+// the result stays within a few ulp), not transcribed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsd {
+
+constexpr int NV = 5;
+
+// ---- boundary conditions -------------------------------------------------------------------
+// Values of BoundaryConditionType (src/BoundaryConditions/BoundaryConditionConfiguration.h:12-19)
+enum : int { BC_WALL = 0, BC_SYMMETRY = 1, BC_MOVING_WALL = 2, BC_SUBSONIC_INLET = 3,
+             BC_SUBSONIC_OUTLET = 4, BC_NATURAL = 5 };
+
+struct BCDev {
+    int type;
+    double vel[3];
+    double p;
+    double rho;
+};
+
+// Image state of a boundary condition across a face normal to `axis` (getDummyValues,
+// BCGeneral.h:186-268).  Every condition in the reference is even in the normal's sign, so the
+// ghost fill (normal -e_axis on both sides, kernel.h:743,760) and the boundary-face states
+// (normal +e_axis, ExplicitRungeKuttaFVM.h:1202,1208) share this one function.
+template <typename R>
+__device__ inline void bc_image(const BCDev& bc, R gamma, int axis, const R (&u)[NV], R (&o)[NV]) {
+    switch (bc.type) {
+    case BC_SYMMETRY:
+        // U - 2 (n.m) n: only the normal momentum changes sign
+        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
+        o[1 + axis] = -u[1 + axis];
+        break;
+    case BC_WALL:
+        o[0] = u[0]; o[1] = -u[1]; o[2] = -u[2]; o[3] = -u[3]; o[4] = u[4];
+        break;
+    case BC_MOVING_WALL: {
+        R rho = u[0];
+        R ir = R(1) / rho;
+        R vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+        R rhoe = u[4] - R(0.5) * rho * (vx * vx + vy * vy + vz * vz);
+        R dx = R(2) * R(bc.vel[0]) - vx, dy = R(2) * R(bc.vel[1]) - vy, dz = R(2) * R(bc.vel[2]) - vz;
+        o[0] = rho; o[1] = rho * dx; o[2] = rho * dy; o[3] = rho * dz;
+        o[4] = rhoe + R(0.5) * rho * (dx * dx + dy * dy + dz * dz);
+        break;
+    }
+    case BC_SUBSONIC_INLET: {
+        R r = R(bc.rho);
+        R vx = R(bc.vel[0]), vy = R(bc.vel[1]), vz = R(bc.vel[2]);
+        o[0] = r; o[1] = r * vx; o[2] = r * vy; o[3] = r * vz;
+        o[4] = R(bc.p) / (gamma - R(1)) + R(0.5) * r * (vx * vx + vy * vy + vz * vz);
+        break;
+    }
+    default:  // Natural, SubsonicOutlet: copy
+        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
+    }
+}
+
+// ---- fast reciprocal / square root ---------------------------------------------------------
+// Hardware estimates refined by Newton steps (double) or used directly (float).  Results are
+// within an ulp or two of the correctly rounded value; the parity tolerance is 1e-12.  Only for
+// operands well inside the normal range (densities, sound speeds, mass fluxes): the estimate
+// overflows for subnormal divisors.
+__device__ inline double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double fdiv(double a, double b) {
+    double r = frcp(b);
+    double q = a * r;
+    double e = __builtin_fma(-b, q, a);
+    return __builtin_fma(e, r, q);
+}
+__device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double fsqrt(double x) { return __builtin_sqrt(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <typename R> __device__ inline R tiny_spread();
+template <> __device__ inline double tiny_spread<double>() { return 3.0 * 2.2250738585072014e-308; }
+template <> __device__ inline float tiny_spread<float>() { return 3.0f * 1.17549435e-38f; }
+
+// ---- reconstruction ------------------------------------------------------------------------
+// Values of Reconstruction / LimiterType (src/Methods/MethodConfiguration.h:5-21)
+enum : int { RC_PWC = 0, RC_LINEAR = 1, RC_ENO2 = 2, RC_LINLIM = 3 };
+enum : int { LIM_BJ = 0, LIM_VENKAT = 1 };
+
+// Geometry of one cell along its ND axes: half sizes and slope denominators.
+template <int ND, typename R>
+struct CellGeo {
+    R hh[ND];   // 0.5 * h (face sample offset, Linear2psLim.h:77-89, ERK:1194-1195)
+    R rc[ND];   // 1 / (x_{i+1} - x_{i-1})  (Linear2PointsStencil.h:153)
+    R rl[ND];   // 1 / (x_i - x_{i-1})      (ENO2PointsStencil.h:209)
+    R rr[ND];   // 1 / (x_{i+1} - x_i)      (ENO2PointsStencil.h:210)
+    R eps2;     // Venkatakrishnan (K dx)^3 (Limiters.h:227-230)
+};
+
+// Reconstruct one cell and return its face states: face[2a] on the minus face of axis a,
+// face[2a+1] on the plus face.  nb[2a], nb[2a+1] are the axis neighbours (x-,x+,y-,y+,z-,z+),
+// the stencil of ComputeSolutionReconstruction (ERK:897-916).
+// With POS set (the HLLC and exact-Godunov paths, which need a real sound speed), a face state
+// with rho <= 0 or 2 rho E <= |m|^2 falls back to the cell value.
+template <int ND, int RECON, int LIM, typename R, bool POS = false>
+__device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
+                                   const CellGeo<ND, R>& geo, R (&face)[2 * ND][NV]) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        if constexpr (RECON == RC_PWC) {  // PiecewiseConstant.h:57
+#pragma unroll
+            for (int f = 0; f < 2 * ND; f++) face[f][v] = c[v];
+        } else {
+            R g[ND];
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                if constexpr (RECON == RC_ENO2) {  // ENO2PointsStencil.h:207-235
+                    R gl = (c[v] - nb[2 * a][v]) * geo.rl[a];
+                    R gr = (nb[2 * a + 1][v] - c[v]) * geo.rr[a];
+                    g[a] = (__builtin_fabs(gl) < __builtin_fabs(gr)) ? gl : gr;
+                } else {  // Linear2PointsStencil.h:147-178
+                    g[a] = (nb[2 * a + 1][v] - nb[2 * a][v]) * geo.rc[a];
+                }
+            }
+            R lim = R(1);
+            if constexpr (RECON == RC_LINLIM) {
+                // Limiter::ComputeLimiterValues (Limiters.h:162-201).  The minimum over the
+                // faces of min(1, l_j) is found by comparing fractions A/D, then divided once.
+                R umax = c[v], umin = c[v];
+#pragma unroll
+                for (int n = 0; n < 2 * ND; n++) {
+                    umax = (umax < nb[n][v]) ? nb[n][v] : umax;
+                    umin = (umin > nb[n][v]) ? nb[n][v] : umin;
+                }
+                if (!(umax - umin < tiny_spread<R>())) {
+                    R bestA = R(1), bestD = R(1);
+                    R inv_un = R(0);
+                    if constexpr (LIM == LIM_VENKAT) inv_un = frcp(R(0.5) * (umax - umin));
+#pragma unroll
+                    for (int a = 0; a < ND; a++) {
+#pragma unroll
+                        for (int s = 0; s < 2; s++) {
+                            R pt = s ? geo.hh[a] : -geo.hh[a];
+                            R d2;
+                            {
+#pragma clang fp contract(off)
+                                // projection (v + g.p) - v, rounded as Linear2psLim.h:77-89
+                                R fv = c[v] + g[a] * pt;
+                                d2 = fv - c[v];
+                            }
+                            if (d2 > R(0) || d2 < R(0)) {
+                                R um = (d2 > R(0)) ? umax : umin;
+                                R A, D;
+                                if constexpr (LIM == LIM_BJ) {  // Limiters.h:214-216
+                                    A = __builtin_fabs(um - c[v]);
+                                    D = __builtin_fabs(d2);
+                                } else {  // Limiters.h:236-249
+                                    R dm = um * inv_un - c[v] * inv_un;
+                                    R dn = d2 * inv_un;
+                                    A = (dm * dm + geo.eps2) + R(2) * dn * dm;
+                                    D = dm * dm + R(2) * dn * dn + dm * dn + geo.eps2;
+                                }
+                                if (A * bestD < bestA * D) { bestA = A; bestD = D; }
+                            }
+                        }
+                    }
+                    // IEEE division: the winning denominator can be subnormal (a variable
+                    // that is zero up to round-off), where a reciprocal estimate overflows
+                    lim = (bestA == bestD) ? R(1) : bestA / bestD;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                R dl = g[a] * (-geo.hh[a]);
+                R dr = g[a] * geo.hh[a];
+                if constexpr (RECON == RC_LINLIM) {  // Linear2psLim.h:17-21
+                    face[2 * a][v] = c[v] + lim * dl;
+                    face[2 * a + 1][v] = c[v] + lim * dr;
+                } else {  // Linear2PointsStencil.h:107-111
+                    face[2 * a][v] = c[v] + dl;
+                    face[2 * a + 1][v] = c[v] + dr;
+                }
+            }
+        }
+    }
+    if constexpr (POS && RECON != RC_PWC) {
+#pragma unroll
+        for (int f = 0; f < 2 * ND; f++) {
+            R q = R(2) * face[f][0] * face[f][4] -
+                  (face[f][1] * face[f][1] + face[f][2] * face[f][2] + face[f][3] * face[f][3]);
+            bool ok = face[f][0] > R(0) && q > R(0);
+#pragma unroll
+            for (int v = 0; v < NV; v++) face[f][v] = ok ? face[f][v] : c[v];
+        }
+    }
+}
+
+// ---- Riemann solvers -------------------------------------------------------------------------
+enum : int { FX_GODUNOV = 0, FX_ROE = 1, FX_HLLC = 3 };
+
+// Internal energy density E - rho|v|^2/2 from the conserved state.  In the FP32 form the
+// difference is redone in double when it is a small part of E (near vacuum, where the float
+// subtraction loses most of its digits).
+__device__ inline double internal_energy(double E, double rho_ke) { return E - rho_ke; }
+__device__ inline float internal_energy(float E, float rho_ke, const float (&U)[NV]) {
+    float e = E - rho_ke;
+    if (e < 0.01f * E) {
+        double m2 = double(U[1]) * U[1] + double(U[2]) * U[2] + double(U[3]) * U[3];
+        e = float(double(U[4]) - 0.5 * m2 / double(U[0]));
+    }
+    return e;
+}
+__device__ inline double internal_energy(double E, double rho_ke, const double (&)[NV]) {
+    return E - rho_ke;
+}
+
+// Roe-Pike flux with Harten's entropy fix (RoeSolverPerfectGasEOS.h:48-134) across a face with
+// normal +e_D.  lam receives |u.n| + c of the Roe average (RoeSolverPerfectGasEOS.h:88,128).
+template <int D, typename R>
+__device__ inline void roe_flux(R gamma, R eps, const R (&UL)[NV], const R (&UR)[NV], R (&F)[NV],
+                                R& lam) {
+    const R gm1 = gamma - R(1);
+    R il = frcp(UL[0]), ir = frcp(UR[0]);
+    R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
+    R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
+    R kl = R(0.5) * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]);
+    R kr = R(0.5) * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+    R el = UL[4] * il, er = UR[4] * ir;
+    // physical fluxes (RoeSolverPerfectGasEOS.h:15-32)
+    R pl = gm1 * internal_energy(UL[4], UL[0] * kl, UL);
+    R pr = gm1 * internal_energy(UR[4], UR[0] * kr, UR);
+    R hl = (el - kl) * gamma + kl;
+    R hr = (er - kr) * gamma + kr;
+    R sl = fsqrt(UL[0]), sr = fsqrt(UR[0]);
+    R isum = frcp(sl + sr);
+    R ql = sl * isum, qr = sr * isum;
+    R v[3] = {ql * vl[0] + qr * vr[0], ql * vl[1] + qr * vr[1], ql * vl[2] + qr * vr[2]};
+    R h = ql * hl + qr * hr;
+    R vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    R phi2 = R(0.5) * gm1 * vv;
+    R c = fsqrt(gm1 * h - phi2);
+    R uw = v[D];
+    R eig = __builtin_fabs(uw) + c;
+    R e = eps * eig;
+    R a1 = __builtin_fabs(uw), a3 = __builtin_fabs(uw + c), a4 = __builtin_fabs(uw - c);
+    if (a1 < e) a1 = (a1 * a1 / e + e) * R(0.5);  // Harten (RoeSolverPerfectGasEOS.h:35-40)
+    if (a3 < e) a3 = (a3 * a3 / e + e) * R(0.5);
+    if (a4 < e) a4 = (a4 * a4 / e + e) * R(0.5);
+    R ic = frcp(c);
+    R Eig1 = a1;
+    R Eiga = (a3 - a4) * R(0.5) * ic;
+    R Eigb = (a3 + a4) * R(0.5) - a1;
+    R R1 = phi2 * UR[0] - gm1 * UR[0] * ((v[0] * vr[0] + v[1] * vr[1] + v[2] * vr[2]) - er);
+    R R2 = phi2 * UL[0] - gm1 * UL[0] * ((v[0] * vl[0] + v[1] * vl[1] + v[2] * vl[2]) - el);
+    R D1 = UR[0] * (vr[D] - v[D]);
+    R D2 = UL[0] * (vl[D] - v[D]);
+    R C2 = ic * ic;
+    // dissipation (RoeSolverPerfectGasEOS.h:96-125) with Qa = (0, n, u.n), Qb = (1, v, h)
+    R vnl = vl[D], vnr = vr[D];
+    R FL[NV], FR[NV];
+    FL[0] = UL[0] * vnl; FR[0] = UR[0] * vnr;
+    FL[1] = FL[0] * vl[0]; FR[1] = FR[0] * vr[0];
+    FL[2] = FL[0] * vl[1]; FR[2] = FR[0] * vr[1];
+    FL[3] = FL[0] * vl[2]; FR[3] = FR[0] * vr[2];
+    FL[1 + D] += pl; FR[1 + D] += pr;
+    FL[4] = vnl * (UL[4] + pl); FR[4] = vnr * (UR[4] + pr);
+    R Qb[NV] = {R(1), v[0], v[1], v[2], h};
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        R qa = (i == 4) ? uw : ((i == 1 + D) ? R(1) : R(0));
+        R diss = Eig1 * (UL[i] - UR[i]) + Eiga * ((R2 - R1) * qa + (D2 - D1) * Qb[i]) +
+                 Eigb * ((R2 - R1) * Qb[i] * C2 + (D2 - D1) * qa);
+        F[i] = R(0.5) * (FL[i] + FR[i] + diss);
+    }
+    lam = eig;
+}
+
+// HLLC flux (Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics, 3rd ed.,
+// sec. 10.4, eqs. 10.37-10.39) with Davis wave-speed bounds (eq. 10.48); lam = max(|SL|,|SR|).
+// Extension with no counterpart in the reference.  Branch-free: the upwind side of the contact
+// is selected first, then its flux and star state are evaluated once.
+template <int D, typename R>
+__device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], R (&F)[NV],
+                                 R& lam) {
+    const R gm1 = gamma - R(1);
+    R il = frcp(UL[0]), ir = frcp(UR[0]);
+    R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
+    R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
+    R pl = gm1 * internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL);
+    R pr = gm1 * internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR);
+    R al = fsqrt(gamma * pl * il), ar = fsqrt(gamma * pr * ir);
+    R ul = vl[D], ur = vr[D];
+    R SL = __builtin_fmin(ul - al, ur - ar);
+    R SR = __builtin_fmax(ul + al, ur + ar);
+    R ml = UL[0] * (SL - ul), mr = UR[0] * (SR - ur);
+    R Ss = fdiv(pr - pl + ul * ml - ur * mr, ml - mr);
+    lam = __builtin_fmax(__builtin_fabs(SL), __builtin_fabs(SR));
+    bool left = Ss >= R(0);
+    R U[NV], vel[3];
+#pragma unroll
+    for (int i = 0; i < NV; i++) U[i] = left ? UL[i] : UR[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) vel[i] = left ? vl[i] : vr[i];
+    R S = left ? SL : SR;
+    R un = left ? ul : ur, p = left ? pl : pr, ik = left ? il : ir;
+    R m = left ? ml : mr;
+    R Fk[NV];
+    Fk[0] = U[0] * un;
+    Fk[1] = Fk[0] * vel[0];
+    Fk[2] = Fk[0] * vel[1];
+    Fk[3] = Fk[0] * vel[2];
+    Fk[1 + D] += p;
+    Fk[4] = un * (U[4] + p);
+    bool supersonic = left ? (S >= R(0)) : (S <= R(0));
+    R coef = fdiv(m, S - Ss);
+    R Us[NV];
+    Us[0] = coef;
+    Us[1] = coef * vel[0];
+    Us[2] = coef * vel[1];
+    Us[3] = coef * vel[2];
+    Us[1 + D] = coef * Ss;
+    Us[4] = coef * (U[4] * ik + (Ss - un) * (Ss + fdiv(p, m)));
+#pragma unroll
+    for (int i = 0; i < NV; i++) F[i] = supersonic ? Fk[i] : Fk[i] + S * (Us[i] - U[i]);
+}
+
+// ---- exact Riemann solver (always double) -----------------------------------------------------
+// f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37).
+__device__ inline double fk_dev(double g, double r, double pk, double ak, double p, double& d) {
+    if (p > pk) {
+        double A = 2.0 / ((g + 1.0) * r);
+        double B = pk * (g - 1.0) / (g + 1.0);
+        double q = __builtin_sqrt(A / (p + B));
+        d = q * (1.0 - 0.5 * (p - pk) / (B + p));
+        return (p - pk) * q;
+    }
+    double pr = p / pk;
+    double res = pow(pr, (g - 1.0) / (2.0 * g)) - 1.0;
+    res *= 2.0 * ak / (g - 1.0);
+    d = pow(pr, -(g + 1.0) / (2.0 * g)) / (r * ak);
+    return res;
+}
+
+struct StarDev {
+    double p, u, rl, rr, SL, SHL, STL, SR, SHR, STR, smax;
+    bool shock_l, shock_r, vacuum;
+};
+
+// Star state by Newton-Raphson from Toro's adaptive initial guess (Toro sec. 4.3, 9.5.1); the
+// reference uses ALGLIB's bounded Levenberg-Marquardt for the same root
+// (GodunovSolverPerfectGasEOS.h:143-185).  Divergent lanes simply run a masked loop.
+__device__ inline StarDev star_state_dev(double g, double rl, double ul, double pl, double rr,
+                                         double ur, double pr) {
+    StarDev s;
+    double al = __builtin_sqrt(g * pl / rl), ar = __builtin_sqrt(g * pr / rr);
+    double du = ur - ul;
+    s.vacuum = (2.0 / (g - 1.0)) * (al + ar) <= du;
+    double p = 0.0;
+    if (!s.vacuum) {
+        double pmin = __builtin_fmin(pl, pr), pmax = __builtin_fmax(pl, pr);
+        double ppv = 0.5 * (pl + pr) - 0.125 * du * (rl + rr) * (al + ar);
+        ppv = __builtin_fmax(0.0, ppv);
+        if (pmax / pmin <= 2.0 && pmin <= ppv && ppv <= pmax) {
+            p = ppv;
+        } else if (ppv < pmin) {
+            double z = (g - 1.0) / (2.0 * g);
+            double plz = pow(pl, z), prz = pow(pr, z);
+            double t = (al + ar - 0.5 * (g - 1.0) * du) / (al / plz + ar / prz);
+            p = pow(t, 1.0 / z);
+        } else {
+            double A = 2.0 / ((g + 1.0) * rl), B = (g - 1.0) / (g + 1.0) * pl;
+            double gl = __builtin_sqrt(A / (ppv + B));
+            A = 2.0 / ((g + 1.0) * rr);
+            B = (g - 1.0) / (g + 1.0) * pr;
+            double gr = __builtin_sqrt(A / (ppv + B));
+            p = (gl * pl + gr * pr - du) / (gl + gr);
+        }
+        if (!(p > 0.0)) p = 1e-12 * pmin;
+        for (int it = 1; it <= 60; it++) {
+            double dl, dr;
+            double f = fk_dev(g, rl, pl, al, p, dl) + fk_dev(g, rr, pr, ar, p, dr) + du;
+            double pn = p - f / (dl + dr);
+            if (!(pn > 0.0)) pn = 0.01 * p;
+            double ch = __builtin_fabs(pn - p) / (pn + p);
+            p = pn;
+            if (ch <= 1e-15) break;
+        }
+    }
+    s.p = p;
+    double d1, d2;
+    double f1 = fk_dev(g, rl, pl, al, p, d1), f2 = fk_dev(g, rr, pr, ar, p, d2);
+    s.u = 0.5 * (ul + ur) + 0.5 * (f2 - f1);
+    double C = (g - 1.0) / (g + 1.0);
+    s.smax = 0.0;
+    double prl = p / pl;
+    s.SL = s.SHL = s.STL = s.SR = s.SHR = s.STR = 0.0;
+    if (p > pl) {
+        s.shock_l = true;
+        s.rl = (prl + C) / (C * prl + 1.0) * rl;
+        s.SL = ul - al * __builtin_sqrt((g + 1) * prl / (2 * g) + (g - 1) / (2 * g));
+        s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SL));
+    } else {
+        s.shock_l = false;
+        s.rl = rl * pow(prl, 1.0 / g);
+        s.SHL = ul - al;
+        s.STL = s.u - al * pow(prl, (g - 1) / (2 * g));
+        s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHL));
+    }
+    double prr = p / pr;
+    if (p > pr) {
+        s.shock_r = true;
+        s.rr = (prr + C) / (C * prr + 1.0) * rr;
+        s.SR = ur + ar * __builtin_sqrt((g + 1) * prr / (2 * g) + (g - 1) / (2 * g));
+        s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SR));
+    } else {
+        s.shock_r = false;
+        s.rr = rr * pow(prr, 1.0 / g);
+        s.SHR = ur + ar;
+        s.STR = s.u + ar * pow(prr, (g - 1) / (2 * g));
+        s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHR));
+    }
+    if (s.vacuum) {  // the fan tails bound the vacuum (Toro sec. 4.6)
+        s.STL = ul + 2.0 * al / (g - 1.0);
+        s.STR = ur - 2.0 * ar / (g - 1.0);
+    }
+    return s;
+}
+
+// Exact Godunov flux: the solution sampled at x/t = 0 (GodunovSolverPerfectGasEOS.h:276-433),
+// tangential velocity from the upwind side of the contact.  Computed in double for both
+// arithmetic forms.
+template <int D, typename R>
+__device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)[NV], R (&F)[NV],
+                                    R& lam) {
+    const double g = double(gammaR);
+    double UL[NV], UR[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) { UL[i] = double(ULr[i]); UR[i] = double(URr[i]); }
+    double il = 1.0 / UL[0], ir = 1.0 / UR[0];
+    double vL[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
+    double vR[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
+    double pL = (g - 1.0) * UL[0] * (UL[4] * il - (vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2]) / 2.0);
+    double pR = (g - 1.0) * UR[0] * (UR[4] * ir - (vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]) / 2.0);
+    double uL = vL[D], uR = vR[D];
+    StarDev st = star_state_dev(g, UL[0], uL, pL, UR[0], uR, pR);
+    double ro, u, p;
+    const double S = 0.0;
+    bool left = st.vacuum ? (S <= 0.5 * (st.STL + st.STR)) : (st.u >= S);
+    if (left) {
+        if (st.shock_l && !st.vacuum) {
+            if (st.SL >= S) { ro = UL[0]; u = uL; p = pL; }
+            else { ro = st.rl; u = st.u; p = st.p; }
+        } else if (st.SHL > S) { ro = UL[0]; u = uL; p = pL; }
+        else if (S > st.STL) {
+            if (st.vacuum) { ro = 0; u = 0; p = 0; } else { ro = st.rl; u = st.u; p = st.p; }
+        } else {
+            double al = __builtin_sqrt(g * pL / UL[0]);
+            double CL = 2.0 / (g + 1) + (g - 1) * (uL - S) / ((g + 1) * al);
+            ro = pow(CL, 2.0 / (g - 1)) * UL[0];
+            u = (al + 0.5 * (g - 1) * uL + S) * (2 / (g + 1));
+            p = pow(CL, 2.0 * g / (g - 1)) * pL;
+        }
+    } else {
+        if (st.shock_r && !st.vacuum) {
+            if (st.SR <= S) { ro = UR[0]; u = uR; p = pR; }
+            else { ro = st.rr; u = st.u; p = st.p; }
+        } else if (st.SHR < S) { ro = UR[0]; u = uR; p = pR; }
+        else if (S < st.STR) {
+            if (st.vacuum) { ro = 0; u = 0; p = 0; } else { ro = st.rr; u = st.u; p = st.p; }
+        } else {
+            double ar = __builtin_sqrt(g * pR / UR[0]);
+            double CR = 2.0 / (g + 1) - (g - 1) * (uR - S) / ((g + 1) * ar);
+            ro = pow(CR, 2.0 / (g - 1)) * UR[0];
+            u = (-ar + 0.5 * (g - 1) * uR + S) * (2 / (g + 1));
+            p = pow(CR, 2.0 * g / (g - 1)) * pR;
+        }
+    }
+    const double* vel = left ? vL : vR;
+    double Vf[3] = {vel[0], vel[1], vel[2]};
+    Vf[D] = vel[D] + (u - vel[D]);
+    double Fd[NV];
+    if (ro == 0.0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) Fd[i] = 0.0;
+    } else {
+        // Uface and F(Uface) (GodunovSolverPerfectGasEOS.h:416-427)
+        double U0 = ro, U1 = ro * Vf[0], U2 = ro * Vf[1], U3 = ro * Vf[2];
+        double U4 = ro * (p / (ro * (g - 1.0)) + 0.5 * (Vf[0] * Vf[0] + Vf[1] * Vf[1] + Vf[2] * Vf[2]));
+        double iro = 1.0 / U0;
+        double vx = U1 * iro, vy = U2 * iro, vz = U3 * iro;
+        double pp = (g - 1.0) * (U4 - U0 * (vx * vx + vy * vy + vz * vz) / 2.0);
+        double vv[3] = {vx, vy, vz};
+        double vn = vv[D];
+        Fd[0] = U0 * vn;
+        Fd[1] = U0 * vn * vx;
+        Fd[2] = U0 * vn * vy;
+        Fd[3] = U0 * vn * vz;
+        Fd[1 + D] += pp;
+        Fd[4] = vn * (U4 + pp);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) F[i] = R(Fd[i]);
+    lam = R(st.smax);
+}
+
+// Face eigenvalue only (the dt pass needs no flux).
+template <int FLUX, int D, typename R>
+__device__ inline R face_eigen(R gamma, const R (&UL)[NV], const R (&UR)[NV]) {
+    if constexpr (FLUX == FX_ROE) {
+        // |u.n| + c of the Roe average (RoeSolverPerfectGasEOS.h:60-88)
+        const R gm1 = gamma - R(1);
+        R il = frcp(UL[0]), ir = frcp(UR[0]);
+        R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
+        R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
+        R kl = R(0.5) * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]);
+        R kr = R(0.5) * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+        R hl = (UL[4] * il - kl) * gamma + kl;
+        R hr = (UR[4] * ir - kr) * gamma + kr;
+        R sl = fsqrt(UL[0]), sr = fsqrt(UR[0]);
+        R isum = frcp(sl + sr);
+        R ql = sl * isum, qr = sr * isum;
+        R v[3] = {ql * vl[0] + qr * vr[0], ql * vl[1] + qr * vr[1], ql * vl[2] + qr * vr[2]};
+        R h = ql * hl + qr * hr;
+        R phi2 = R(0.5) * gm1 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        return __builtin_fabs(v[D]) + fsqrt(gm1 * h - phi2);
+    } else if constexpr (FLUX == FX_HLLC) {
+        const R gm1 = gamma - R(1);
+        R il = frcp(UL[0]), ir = frcp(UR[0]);
+        R ul = UL[1 + D] * il, ur = UR[1 + D] * ir;
+        R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
+        R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
+        R pl = gm1 * internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL);
+        R pr = gm1 * internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR);
+        R al = fsqrt(gamma * pl * il), ar = fsqrt(gamma * pr * ir);
+        R SL = __builtin_fmin(ul - al, ur - ar);
+        R SR = __builtin_fmax(ul + al, ur + ar);
+        return __builtin_fmax(__builtin_fabs(SL), __builtin_fabs(SR));
+    } else {
+        R F[NV], lam;
+        godunov_flux<D, R>(gamma, UL, UR, F, lam);
+        return lam;
+    }
+}
+
+template <int FLUX, int D, typename R>
+__device__ inline void face_flux(R gamma, R eps, const R (&UL)[NV], const R (&UR)[NV], R (&F)[NV],
+                                 R& lam) {
+    if constexpr (FLUX == FX_ROE) roe_flux<D, R>(gamma, eps, UL, UR, F, lam);
+    else if constexpr (FLUX == FX_HLLC) hllc_flux<D, R>(gamma, UL, UR, F, lam);
+    else godunov_flux<D, R>(gamma, UL, UR, F, lam);
+}
+
+}  // namespace tsd

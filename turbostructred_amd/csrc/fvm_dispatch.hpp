@@ -1,0 +1,27 @@
+// fvm_dispatch.hpp -- table of the compiled kernel instances, filled per (dimension, flux)
+// translation unit and looked up by the host runtime at ts_create.
+#pragma once
+
+#include "fvm_kernels.hpp"
+
+namespace tsd {
+
+using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
+
+// dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3
+struct KernelSet {
+    LaunchFn dt = nullptr, s1e = nullptr, s1r = nullptr, s2 = nullptr, s3 = nullptr;
+};
+
+// reconstruction variants: 0 PWC, 1 linear, 2 ENO2, 3 limited linear BJ, 4 limited linear Venkat
+bool get_kernels_d1_f0(int rv, int precision, KernelSet* ks);
+bool get_kernels_d1_f1(int rv, int precision, KernelSet* ks);
+bool get_kernels_d1_f3(int rv, int precision, KernelSet* ks);
+bool get_kernels_d2_f0(int rv, int precision, KernelSet* ks);
+bool get_kernels_d2_f1(int rv, int precision, KernelSet* ks);
+bool get_kernels_d2_f3(int rv, int precision, KernelSet* ks);
+bool get_kernels_d3_f0(int rv, int precision, KernelSet* ks);
+bool get_kernels_d3_f1(int rv, int precision, KernelSet* ks);
+bool get_kernels_d3_f3(int rv, int precision, KernelSet* ks);
+
+}  // namespace tsd

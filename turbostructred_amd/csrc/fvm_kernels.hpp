@@ -1,0 +1,413 @@
+// fvm_kernels.hpp -- fused finite-volume stage kernels for gfx950 (1-D and 2-D marching form).
+//
+// One wavefront owns a strip of 64 columns along x (60 output columns plus a halo of two
+// lanes on each side) and marches along y over `ty` output rows, keeping a three-row window of
+// the state in registers.  x-neighbours come from the adjacent lanes with DPP wave shifts, so
+// no LDS is used; every face flux is evaluated once, by the lane on its plus side, and the
+// residual of a cell is accumulated in registers in x-then-y order (ERK:1232,1312) before the
+// stage update is applied and the row is stored with coalesced SoA writes.
+//
+// Modes: MODE_DT computes the spectral radius of every cell from the face eigenvalues, takes
+// min(V/SR) over the grid (wave shuffle, LDS across waves, one atomic per block) and the last
+// block to finish turns it into dt (ERK:1427-1437).  The stage modes apply forward Euler / SSP
+// stage 1, SSP stage 2 and SSP stage 3 with that dt.
+#pragma once
+
+#include "fvm_device.hpp"
+
+namespace tsd {
+
+enum : int { MODE_DT = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3 };
+constexpr int STRIP = 60;   // output columns per wavefront
+constexpr int ZPLANES = 8;  // output z-planes per 3-D workgroup (one wavefront each)
+constexpr int MAX_BC = 8;
+
+// Device-resident control block of one solve: the step bookkeeping of StepInfo (kernel.h:56) and
+// the termination rule of Kernel::Run (kernel.h:1834-1846), evaluated on the device so that a
+// chunk of steps can be issued without host round trips.
+struct StepCtl {
+    double time, dt, t_max, next_snap;
+    long long iter;
+    int max_steps, steps_done;
+    int active;  // 1 while steps remain to be taken
+    int go;      // 1 if the current step runs (set by the dt pass)
+    int error;   // 1 = dt not finite and positive
+    int pad0;
+    unsigned long long min_bits;  // running min of V/SR as ordered bits
+    unsigned int blocks_done, pad1;
+    double* dt_log;
+    long long dt_log_base, dt_log_cap;  // dt of iteration `it` goes to dt_log[it - base]
+};
+
+struct GridArgs {
+    int nd;
+    int n[3];
+    int periodic[3];
+    const double* h[3];   // cell sizes, ghost layer included (n + 2 entries)
+    const double* rc[3];  // 1/(x_{i+1} - x_{i-1}), interior cells
+    const double* rl[3];  // 1/(x_i - x_{i-1})
+    const double* rr[3];  // 1/(x_{i+1} - x_i)
+    double eps2;          // Venkatakrishnan eps^2 (uniform grids)
+    double gamma, eps, cfl;
+    BCDev bc[MAX_BC];
+    int side_marker[6];
+    const uint8_t* face_marker[6];
+};
+
+struct KArgs {
+    GridArgs g;
+    const void* uin;   // stencil input planes (TIn)
+    const double* un;  // U^n planes (SSP stages 2, 3)
+    void* uout;        // output planes (TOut)
+    size_t plane;      // cells per plane
+    StepCtl* ctl;
+    int ty;            // output rows per tile
+    int nstrips;       // strips along x
+    int ntiles;        // strips * row blocks
+};
+
+// ---- wave-level helpers -------------------------------------------------------------------------
+// lane i receives lane i-1 (wave_shr:1) / lane i+1 (wave_shl:1); the end lanes are halo lanes.
+__device__ inline double dpp_from_left(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double dpp_from_right(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float dpp_from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ inline float dpp_from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+// tiles so that neighbouring tiles (which share halo rows/columns) share an L2.
+__device__ inline int xcd_remap(int b, int nb) {
+    int q = nb >> 3, r = nb & 7;
+    int x = b & 7, l = b >> 3;
+    return x * q + (x < r ? x : r) + l;
+}
+
+// Map a possibly out-of-range index to its source cell: periodic wrap, or the mirror image
+// of the ghost fill (ghost i = -l takes cell l-1, kernel.h:734-735, 751-752).
+__device__ inline void map_index(int x, int n, int periodic, int& xs, int& side) {
+    side = 0;
+    if (periodic) {
+        xs = x % n;
+        if (xs < 0) xs += n;
+        return;
+    }
+    if (x < 0) {
+        xs = -x - 1;
+        side = -1;
+    } else if (x >= n) {
+        xs = 2 * n - 1 - x;
+        side = 1;
+    } else {
+        xs = x;
+    }
+    xs = xs < 0 ? 0 : (xs >= n ? n - 1 : xs);
+}
+
+__device__ inline int bc_marker(const GridArgs& g, int side, int face_index) {
+    const uint8_t* fm = g.face_marker[side];
+    return fm ? int(fm[face_index]) : g.side_marker[side];
+}
+
+__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+// V/SR of one cell; a NaN or negative spectral radius (non-physical state) becomes 0 so that
+// the step fails with dt = 0 instead of silently skipping the cell in the minimum.
+__device__ inline double dt_candidate(double vol, double sr) {
+    double q = vol / sr;
+    return (q > 0.0) ? q : 0.0;
+}
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// ---- the 1-D / 2-D stage kernel -----------------------------------------------------------------
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+__global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
+    static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
+    StepCtl* ctl = a.ctl;
+    __shared__ double red[4];
+    if constexpr (MODE == MODE_DT) {
+        if (!ctl->active) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
+            return;
+        }
+    } else {
+        if (!ctl->go) return;
+    }
+    const GridArgs& g = a.g;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
+    const bool tile_ok = tile < a.ntiles;
+    if constexpr (MODE != MODE_DT) {
+        if (!tile_ok) return;  // no block-level synchronisation in the stage modes
+    }
+    const int strip = tile_ok ? tile % a.nstrips : 0;
+    const int yb = tile_ok ? tile / a.nstrips : 0;
+
+    const int nx = g.n[0];
+    const int ny = (DIM > 1) ? g.n[1] : 1;
+    const int xg = strip * STRIP - 2 + lane;
+    int xs, xside;
+    map_index(xg, nx, g.periodic[0], xs, xside);
+    const bool out_lane = tile_ok && lane >= 2 && lane < 2 + STRIP && xg < nx;
+
+    const R gamma = R(g.gamma), eps = R(g.eps);
+    const double dt = (MODE == MODE_DT) ? 0.0 : ctl->dt;
+
+    CellGeo<DIM, R> geo;
+    const double hx = g.h[0][xs + 1];
+    geo.hh[0] = R(0.5 * hx);
+    geo.rc[0] = R(g.rc[0][xs]);
+    geo.rl[0] = R(g.rl[0][xs]);
+    geo.rr[0] = R(g.rr[0][xs]);
+    geo.eps2 = R(g.eps2);
+
+    const TIn* __restrict__ uin = (const TIn*)a.uin;
+    const size_t P = a.plane;
+
+    // load one row of the stencil input (row index may be a ghost row)
+    auto load_row = [&](int r, R (&u)[NV]) {
+        int rs = 0, yside = 0;
+        if constexpr (DIM > 1) map_index(r, ny, g.periodic[1], rs, yside);
+        size_t off = (size_t)rs * nx + xs;
+#pragma unroll
+        for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+        if (xside != 0) {
+            R t[NV];
+            bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, rs)], gamma, 0, u, t);
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = t[v];
+        } else if (yside != 0) {
+            R t[NV];
+            bc_image(g.bc[bc_marker(g, yside < 0 ? 2 : 3, xs)], gamma, 1, u, t);
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = t[v];
+        }
+    };
+
+    double block_min = __builtin_inf();
+
+    if constexpr (DIM == 1) {
+        // ---- one row ----
+        R c[NV], nb[2][NV], face[2][NV];
+        load_row(0, c);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            nb[0][v] = dpp_from_left(c[v]);
+            nb[1][v] = dpp_from_right(c[v]);
+        }
+        reconstruct<1, RECON, LIM, R, FLUX != FX_ROE>(c, nb, geo, face);
+        R UL[NV], UR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            UL[v] = dpp_from_left(face[1][v]);
+            UR[v] = face[0][v];
+        }
+        if (!g.periodic[0]) {
+            if (xg == 0) bc_image(g.bc[bc_marker(g, 0, 0)], gamma, 0, UR, UL);
+            if (xg == nx) bc_image(g.bc[bc_marker(g, 1, 0)], gamma, 0, UL, UR);
+        }
+        const R Sx = R(1);
+        if constexpr (MODE == MODE_DT) {
+            R lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+            R lam_r = dpp_from_right(lam);
+            double sr;
+            {
+#pragma clang fp contract(off)
+                sr = double(Sx * lam) + double(Sx * lam_r);
+            }
+            if (out_lane) block_min = dt_candidate(hx, sr);
+            if (tile_ok && xg == nx && lane >= 2 && lane <= 62)
+                block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1], double(Sx * lam)));
+        } else {
+            R F[NV], lam;
+            face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
+            const double ivol = 1.0 / hx;
+            TOut* __restrict__ uo = (TOut*)a.uout;
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                R Fr = dpp_from_right(F[v]);
+                R res = -((Fr - F[v]) * Sx);
+                double u = double(c[v]);
+                double q = u + dt * (double(res) * ivol);
+                if constexpr (MODE == MODE_S2) q = 0.75 * a.un[v * P + xs] + 0.25 * q;
+                if constexpr (MODE == MODE_S3) q = (1.0 / 3.0) * a.un[v * P + xs] + (2.0 / 3.0) * q;
+                if (out_lane) uo[v * P + xs] = TOut(q);
+            }
+        }
+    } else {
+        // ---- march along y ----
+        const int y0 = yb * a.ty;
+        const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
+        R Um[NV], U0[NV], Up[NV];
+        R ytop[NV];  // +y face state of the previously reconstructed row
+        R gbot[NV];  // y-flux through the bottom face of the pending row
+        R dfx[NV];   // x part of the residual of the pending row
+        double srp = 0.0;  // spectral radius of the pending row without its top face
+        R lam_gb = R(0);
+        load_row(y0 - 2, Um);
+        load_row(y0 - 1, U0);
+        const int nit = yend - y0 + 2;
+        TOut* __restrict__ uo = (TOut*)a.uout;
+        for (int t = 0; t < nit; t++) {
+            const int rc = y0 - 1 + t;  // row being reconstructed
+            load_row(rc + 1, Up);
+            int rcs, rcside;
+            map_index(rc, ny, g.periodic[1], rcs, rcside);
+            const double hy = g.h[1][rcs + 1];
+            geo.hh[1] = R(0.5 * hy);
+            geo.rc[1] = R(g.rc[1][rcs]);
+            geo.rl[1] = R(g.rl[1][rcs]);
+            geo.rr[1] = R(g.rr[1][rcs]);
+            R nb[4][NV], face[4][NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                nb[0][v] = dpp_from_left(U0[v]);
+                nb[1][v] = dpp_from_right(U0[v]);
+                nb[2][v] = Um[v];
+                nb[3][v] = Up[v];
+            }
+            reconstruct<2, RECON, LIM, R, FLUX != FX_ROE>(U0, nb, geo, face);
+            if (t >= 1) {
+                // face between rows rc-1 and rc
+                R GL[NV], GR[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) { GL[v] = ytop[v]; GR[v] = face[2][v]; }
+                if (!g.periodic[1]) {
+                    if (rc == 0) bc_image(g.bc[bc_marker(g, 2, xs)], gamma, 1, GR, GL);
+                    if (rc == ny) bc_image(g.bc[bc_marker(g, 3, xs)], gamma, 1, GL, GR);
+                }
+                const R Sy = R(hx);  // face area of a y-face (ERK:1268)
+                if constexpr (MODE == MODE_DT) {
+                    R lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+                    if (t >= 2) {
+                        double sr;
+                        {
+#pragma clang fp contract(off)
+                            sr = srp + double(Sy * lam);
+                        }
+                        if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc], sr));
+                    }
+                    if (rc == ny && out_lane)  // top ghost row: only this face (ERK:1305)
+                        block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][ny + 1], double(Sy * lam)));
+                    lam_gb = lam;
+                } else {
+                    R G[NV], lam;
+                    face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
+                    if (t >= 2) {
+                        // finalize row rc-1 = the row held in Um
+                        const int rf = rc - 1;
+                        const double ivol = 1.0 / (hx * g.h[1][rf + 1]);
+                        const size_t off = (size_t)rf * nx + xs;
+#pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            R res = dfx[v] - (G[v] - gbot[v]) * Sy;
+                            double q = double(Um[v]) + dt * (double(res) * ivol);
+                            if constexpr (MODE == MODE_S2) q = 0.75 * a.un[v * P + off] + 0.25 * q;
+                            if constexpr (MODE == MODE_S3) q = (1.0 / 3.0) * a.un[v * P + off] + (2.0 / 3.0) * q;
+                            if (out_lane) uo[v * P + off] = TOut(q);
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; v++) gbot[v] = G[v];
+                }
+                if (rc < yend) {
+                    // x-faces of row rc: each lane evaluates its minus face
+                    R UL[NV], UR[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        UL[v] = dpp_from_left(face[1][v]);
+                        UR[v] = face[0][v];
+                    }
+                    if (!g.periodic[0]) {
+                        if (xg == 0) bc_image(g.bc[bc_marker(g, 0, rcs)], gamma, 0, UR, UL);
+                        if (xg == nx) bc_image(g.bc[bc_marker(g, 1, rcs)], gamma, 0, UL, UR);
+                    }
+                    const R Sx = R(hy);  // face area of an x-face (ERK:1189)
+                    if constexpr (MODE == MODE_DT) {
+                        R lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+                        R lam_r = dpp_from_right(lam);
+                        {
+#pragma clang fp contract(off)
+                            srp = double(Sx * lam) + double(Sx * lam_r);
+                            srp = srp + double(Sy * lam_gb);
+                        }
+                        if (tile_ok && xg == nx && lane >= 2 && lane <= 62)  // right ghost column (ERK:1225)
+                            block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy, double(Sx * lam)));
+                    } else {
+                        R F[NV], lam;
+                        face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
+#pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            R Fr = dpp_from_right(F[v]);
+                            dfx[v] = -((Fr - F[v]) * Sx);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                ytop[v] = face[3][v];
+                Um[v] = U0[v];
+                U0[v] = Up[v];
+            }
+        }
+    }
+
+    if constexpr (MODE == MODE_DT) {
+        // wave min, then block min, then one atomic per block
+        double m = block_min;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < nwaves; w++) m = __builtin_fmin(m, red[w]);
+            atomicMin(&ctl->min_bits, dbits(m));
+            __threadfence();
+            unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
+            if (prev == gridDim.x - 1) {
+                __threadfence();
+                double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
+                double dtn = mn * g.cfl;  // ERK:1430
+                double t0 = ctl->time;
+                if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;  // ERK:1433
+                if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
+                    ctl->error = 1;
+                    ctl->go = 0;
+                    ctl->active = 0;
+                } else {
+                    double t1 = t0 + dtn;
+                    ctl->dt = dtn;
+                    ctl->time = t1;
+                    long long it = ctl->iter;
+                    ctl->iter = it + 1;
+                    int sd = ctl->steps_done + 1;
+                    ctl->steps_done = sd;
+                    long long li = it - ctl->dt_log_base;
+                    if (ctl->dt_log && li >= 0 && li < ctl->dt_log_cap) ctl->dt_log[li] = dtn;
+                    ctl->go = 1;
+                    bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
+                                (sd >= ctl->max_steps);
+                    ctl->active = stop ? 0 : 1;
+                }
+                atomicExch(&ctl->min_bits, ~0ull);
+                atomicExch(&ctl->blocks_done, 0u);
+            }
+        }
+    }
+}
+
+}  // namespace tsd

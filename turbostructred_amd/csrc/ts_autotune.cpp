@@ -1,0 +1,96 @@
+// ts_autotune.cpp -- the tile-shape autotuner and its process-wide table.
+//
+// The only free tiling parameter of the marching kernels is the number of output rows per tile
+// (cells per thread along the march).  Results never depend on it: every cell's arithmetic is
+// the same whatever the tiling, so the table may be keyed by problem size alone.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "ts_internal.hpp"
+
+namespace {
+using Key = std::tuple<int, int, int, int, int, int>;  // nd, riemann, recon variant, precision, stages, size bucket
+std::mutex g_mu;
+std::map<Key, int> g_table;
+
+int size_bucket(const ts_config& c) {
+    long long n = (long long)c.nx * c.ny * c.nz;
+    int b = 0;
+    while (n > 1) { n >>= 1; b++; }
+    return b;
+}
+
+Key key_of(const ts_config& c) {
+    return Key(c.ndims, c.riemann, tsr::recon_variant(c), c.precision, c.rk_stages, size_bucket(c));
+}
+}  // namespace
+
+namespace tsr {
+int tuned_rows_per_tile(const ts_config& c) {
+    if (c.ndims == 1) return 1;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_table.find(key_of(c));
+        if (it != g_table.end()) return it->second;
+    }
+    // default: enough tiles to fill 256 CUs several times over, at most 32 rows
+    long long strips = (c.nx + 59) / 60;
+    long long rows = c.ndims == 2 ? c.ny : (long long)c.ny * ((c.nz + 7) / 8);
+    int ty = 32;
+    while (ty > 4 && strips * ((rows + ty - 1) / ty) < 4096) ty /= 2;
+    return ty;
+}
+}  // namespace tsr
+
+extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
+    if (!cfg || cfg->ndims == 1) {
+        if (chosen) *chosen = 1;
+        if (best_ms) *best_ms = 0;
+        return cfg ? TS_OK : TS_ERR_INVALID;
+    }
+    const int cands[] = {4, 8, 16, 32, 64};
+    size_t nst = ts_state_size_cfg(cfg);
+    std::vector<double> state(nst, 0.0);
+    // a quiescent uniform state: arithmetic cost does not depend on the data
+    for (size_t i = 0; i < nst / 5; i++) {
+        state[i * 5 + 0] = 1.0;
+        state[i * 5 + 4] = 2.5;
+    }
+    int best = -1;
+    double bms = 1e300;
+    for (int ty : cands) {
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            g_table[key_of(*cfg)] = ty;
+        }
+        ts_ctx* c = nullptr;
+        if (ts_create(cfg, &c) != TS_OK) continue;
+        ts_set_state(c, state.data(), nst);
+        ts_step_info info;
+        ts_step(c, 2, 1e300, 0, &info);  // warm-up
+        ts_enable_kernel_timing(c, 1);
+        ts_step(c, 5, 1e300, 0, &info);
+        double ms = 0;
+        long long n = 0;
+        ts_get_kernel_timing(c, 4, &ms, &n);
+        ts_destroy(c);
+        if (n > 0 && ms < bms) {
+            bms = ms;
+            best = ty;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (best < 0) {
+        g_table.erase(key_of(*cfg));
+        return TS_ERR_UNSUPPORTED;
+    }
+    g_table[key_of(*cfg)] = best;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = bms / 5.0;
+    return TS_OK;
+}

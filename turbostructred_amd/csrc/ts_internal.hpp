@@ -1,0 +1,78 @@
+// ts_internal.hpp -- host-side state of a solver context (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ts_fvm.h"
+#include "fvm_dispatch.hpp"
+
+struct ts_ctx {
+    ts_config cfg;
+    std::string err;
+    int nd = 1;
+    int n[3] = {1, 1, 1};
+    size_t ncell = 0;
+    hipStream_t stream = nullptr;
+
+    // device state: dU[0] is U^n (double); dU[1], dU[2] are stage arrays (double, or float in
+    // the FP32-residual SSP form).  Forward Euler ping-pongs between dU[0] and dU[1]; cur is the
+    // buffer holding the current state.
+    void* dU[3] = {nullptr, nullptr, nullptr};
+    size_t stage_elem = 8;
+    int cur = 0;
+
+    // geometry on the device
+    std::vector<double*> dbufs;  // everything to free
+    tsd::GridArgs grid{};
+    std::vector<double> X[3], H[3];
+    std::vector<uint8_t> hmarker[6];  // host copies of per-face markers (ghost images on download)
+
+    tsd::StepCtl* dctl = nullptr;
+    tsd::StepCtl* hctl = nullptr;  // pinned mirror
+    double* dt_log = nullptr;
+    long long dt_log_cap = 0, dt_log_base = 0;
+
+    tsd::KernelSet ks;
+    int ty = 16;
+    int nstrips = 1, ntiles = 1;
+    dim3 grid_dim, block_dim;
+
+    // host view of the step bookkeeping
+    double h_time = 0, h_dt = 0;
+    long long h_iter = 0;
+    int h_steps_done = 0;
+
+    // kernel timing
+    bool timing = false;
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    long long t_n[5] = {0, 0, 0, 0, 0};
+    std::vector<hipEvent_t> events;
+
+    // hipGraph of a chunk of steps (built lazily per (chunk length, start buffer))
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+    bool use_graphs = true;
+
+    std::vector<double> host_stage;  // SoA staging for uploads / downloads
+    size_t device_bytes = 0;
+};
+
+namespace tsr {
+std::string hip_msg(hipError_t e, const char* what);
+void build_axis(const ts_config& c, int a, std::vector<double>& X, std::vector<double>& H);
+bool uniform_axis(const std::vector<double>& H);
+int recon_variant(const ts_config& c);
+bool lookup_kernels(int dim, int flux, int rv, int precision, tsd::KernelSet* ks);
+void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, double* o);
+int tuned_rows_per_tile(const ts_config& c);
+}  // namespace tsr
+
+size_t ts_state_size_cfg(const ts_config* c);
+int init_device_ctx(ts_ctx* c, const ts_config* cfg);
+void destroy_device_ctx(ts_ctx* c);
+void aos_to_soa(const ts_ctx* c, const double* aos, double* soa);
+void soa_to_aos(const ts_ctx* c, const double* soa, double* aos);
+int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap);

@@ -1,0 +1,330 @@
+// ts_runtime.cpp -- host side of the C ABI (include/ts_fvm.h) for one solve per context.
+//
+// Replaces Kernel::Init / ExplicitRungeKuttaFVM::InitializeMethod (kernel.h:178-281,
+// ERK:39-75), SetInitialConditions' upload, IterationStep (ERK:1440-1491) and the termination
+// rule of Kernel::Run (kernel.h:1834-1846).  State lives on the device as five SoA planes of
+// interior cells; the reference's AoS array with ghost layers only exists at the boundary
+// (ts_set_state / ts_get_state).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ts_fvm.h"
+#include "fvm_dispatch.hpp"
+#include "ts_internal.hpp"
+
+using namespace tsd;
+
+namespace tsr {
+
+std::string hip_msg(hipError_t e, const char* what) {
+    return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+// Build per-axis geometry exactly as grid.h:124-218 does for a uniform block (running sums of
+// the step), or take the caller's arrays.
+void build_axis(const ts_config& c, int a, std::vector<double>& X, std::vector<double>& H) {
+    int n = a == 0 ? c.nx : (a == 1 ? c.ny : c.nz);
+    double L = a == 0 ? c.lx : (a == 1 ? c.ly : c.lz);
+    int g = c.ghost;
+    X.assign(n + 2 * g, 0.0);
+    H.assign(n + 2 * g, 0.0);
+    if (c.coord[a] && c.h[a]) {
+        for (int i = 0; i < n + 2 * g; i++) {
+            X[i] = c.coord[a][i];
+            H[i] = c.h[a][i];
+        }
+        return;
+    }
+    double stp = L / n;
+    double xc = -(g * stp) + 0.5 * stp;
+    int k = 0;
+    for (; k < g; k++) { X[k] = xc; H[k] = stp; xc += stp; }
+    for (int i = 0; i < n; i++) { X[k + i] = xc; H[k + i] = stp; xc += 0.5 * stp * (1.0 + 1.0); }
+    k += n;
+    for (int i = 0; i < g; i++) { X[k + i] = xc; H[k + i] = stp; xc += stp; }
+}
+
+bool uniform_axis(const std::vector<double>& H) {
+    for (double h : H)
+        if (h != H[0]) return false;
+    return true;
+}
+
+int validate(const ts_config* c, std::string& err) {
+    if (!c) { err = "null config"; return TS_ERR_INVALID; }
+    if (c->ndims < 1 || c->ndims > 3) { err = "ndims must be 1, 2 or 3"; return TS_ERR_INVALID; }
+    if (c->ghost != 1) { err = "DummyLayerSize (ghost) must be 1"; return TS_ERR_UNSUPPORTED; }
+    int n[3] = {c->nx, c->ny, c->nz};
+    for (int a = 0; a < c->ndims; a++)
+        if (n[a] < 2) { err = "every active axis needs at least 2 cells"; return TS_ERR_INVALID; }
+    for (int a = c->ndims; a < 3; a++)
+        if (n[a] != 1) { err = "inactive axes must have 1 cell"; return TS_ERR_INVALID; }
+    if (!(c->gamma > 1.0)) { err = "gamma must exceed 1"; return TS_ERR_INVALID; }
+    if (!(c->cfl > 0.0)) { err = "CFL must be positive"; return TS_ERR_INVALID; }
+    if (c->riemann != TS_RIEMANN_ROE && c->riemann != TS_RIEMANN_GODUNOV && c->riemann != TS_RIEMANN_HLLC) {
+        err = "unknown Riemann solver";
+        return TS_ERR_INVALID;
+    }
+    if (c->recon < 0 || c->recon > 3) { err = "unknown reconstruction"; return TS_ERR_INVALID; }
+    if (c->limiter < 0 || c->limiter > 1) { err = "unknown limiter"; return TS_ERR_INVALID; }
+    if (c->rk_stages != 1 && c->rk_stages != 3) { err = "rk_stages must be 1 or 3"; return TS_ERR_INVALID; }
+    if (c->precision != TS_FP64 && c->precision != TS_FP32_RESIDUAL) { err = "unknown precision"; return TS_ERR_INVALID; }
+    if (c->n_bc < 0 || c->n_bc > TS_MAX_BC) { err = "n_bc out of range"; return TS_ERR_INVALID; }
+    for (int m = 0; m < c->n_bc; m++)
+        if (c->bc[m].type < TS_BC_WALL || c->bc[m].type > TS_BC_NATURAL) { err = "unknown BC type"; return TS_ERR_INVALID; }
+    for (int a = 0; a < c->ndims; a++) {
+        if (c->periodic[a]) continue;
+        for (int s = 0; s < 2; s++) {
+            int side = 2 * a + s;
+            if (c->face_marker[side]) continue;
+            if (c->side_marker[side] < 0 || c->side_marker[side] >= c->n_bc) {
+                err = "side marker refers to a missing boundary condition";
+                return TS_ERR_INVALID;
+            }
+        }
+    }
+    return TS_OK;
+}
+
+int recon_variant(const ts_config& c) {
+    if (c.recon == TS_RECON_LINEAR_LIM) return c.limiter == TS_LIMITER_VENKAT ? 4 : 3;
+    return c.recon;
+}
+
+bool lookup_kernels(int dim, int flux, int rv, int precision, KernelSet* ks) {
+    if (dim == 1) {
+        if (flux == 0) return get_kernels_d1_f0(rv, precision, ks);
+        if (flux == 1) return get_kernels_d1_f1(rv, precision, ks);
+        if (flux == 3) return get_kernels_d1_f3(rv, precision, ks);
+    } else if (dim == 2) {
+        if (flux == 0) return get_kernels_d2_f0(rv, precision, ks);
+        if (flux == 1) return get_kernels_d2_f1(rv, precision, ks);
+        if (flux == 3) return get_kernels_d2_f3(rv, precision, ks);
+    } else {
+        if (flux == 0) return get_kernels_d3_f0(rv, precision, ks);
+        if (flux == 1) return get_kernels_d3_f1(rv, precision, ks);
+        if (flux == 3) return get_kernels_d3_f3(rv, precision, ks);
+    }
+    return false;
+}
+
+// Host copy of the BC images used for the ghost layers of ts_get_state.
+void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, double* o) {
+    for (int v = 0; v < 5; v++) o[v] = u[v];
+    switch (bc.type) {
+    case TS_BC_SYMMETRY: o[1 + axis] = -u[1 + axis]; break;
+    case TS_BC_WALL: o[1] = -u[1]; o[2] = -u[2]; o[3] = -u[3]; break;
+    case TS_BC_MOVING_WALL: {
+        double r = u[0], vx = u[1] / r, vy = u[2] / r, vz = u[3] / r;
+        double re = u[4] - 0.5 * r * (vx * vx + vy * vy + vz * vz);
+        double dx = 2 * bc.velocity[0] - vx, dy = 2 * bc.velocity[1] - vy, dz = 2 * bc.velocity[2] - vz;
+        o[1] = r * dx; o[2] = r * dy; o[3] = r * dz;
+        o[4] = re + 0.5 * r * (dx * dx + dy * dy + dz * dz);
+        break;
+    }
+    case TS_BC_SUBSONIC_INLET: {
+        const double* V = bc.velocity;
+        o[0] = bc.density; o[1] = bc.density * V[0]; o[2] = bc.density * V[1]; o[3] = bc.density * V[2];
+        o[4] = bc.pstatic / (g - 1) + 0.5 * bc.density * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+        break;
+    }
+    default: break;
+    }
+}
+
+}  // namespace tsr
+
+using namespace tsr;
+
+size_t ts_state_size_cfg(const ts_config* c) {
+    size_t nxa = c->nx + 2, nya = c->ndims > 1 ? c->ny + 2 : 1, nza = c->ndims > 2 ? c->nz + 2 : 1;
+    return nxa * nya * nza * 5;
+}
+
+#define HIPCHK(ctx, call)                                        \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) {                                  \
+            (ctx)->err = hip_msg(e_, #call);                     \
+            return TS_ERR_HIP;                                   \
+        }                                                        \
+    } while (0)
+
+static int ctx_fail(ts_ctx* c, int code, const std::string& m) {
+    c->err = m;
+    return code;
+}
+
+extern "C" {
+
+int ts_create(const ts_config* cfg, ts_ctx** out) {
+    if (!out) return TS_ERR_INVALID;
+    *out = nullptr;
+    std::string err;
+    int rc = validate(cfg, err);
+    if (rc) {
+        fprintf(stderr, "ts_create: %s\n", err.c_str());
+        return rc;
+    }
+    ts_ctx* c = new ts_ctx();
+    c->cfg = *cfg;
+    for (int s = 0; s < 6; s++) c->cfg.face_marker[s] = nullptr;
+    for (int a = 0; a < 3; a++) { c->cfg.coord[a] = nullptr; c->cfg.h[a] = nullptr; }
+    c->nd = cfg->ndims;
+    c->n[0] = cfg->nx;
+    c->n[1] = cfg->ny;
+    c->n[2] = cfg->nz;
+    c->ncell = (size_t)c->n[0] * c->n[1] * c->n[2];
+    int r = init_device_ctx(c, cfg);
+    if (r != TS_OK) {
+        fprintf(stderr, "ts_create: %s\n", c->err.c_str());
+        destroy_device_ctx(c);
+        delete c;
+        return r;
+    }
+    *out = c;
+    return TS_OK;
+}
+
+int ts_destroy(ts_ctx* c) {
+    if (!c) return TS_ERR_INVALID;
+    destroy_device_ctx(c);
+    delete c;
+    return TS_OK;
+}
+
+const char* ts_last_error(const ts_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t ts_state_size(const ts_ctx* c) {
+    if (!c) return 0;
+    size_t nxa = c->n[0] + 2, nya = c->nd > 1 ? c->n[1] + 2 : 1, nza = c->nd > 2 ? c->n[2] + 2 : 1;
+    return nxa * nya * nza * 5;
+}
+
+int ts_set_state(ts_ctx* c, const double* aos, size_t n) {
+    if (!c || !aos) return TS_ERR_INVALID;
+    if (n != ts_state_size(c)) return ctx_fail(c, TS_ERR_INVALID, "state size mismatch");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    aos_to_soa(c, aos, c->host_stage.data());
+    HIPCHK(c, hipMemcpyAsync(c->dU[0], c->host_stage.data(), c->ncell * 5 * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    return TS_OK;
+}
+
+int ts_get_state(ts_ctx* c, double* aos, size_t n) {
+    if (!c || !aos) return TS_ERR_INVALID;
+    if (n != ts_state_size(c)) return ctx_fail(c, TS_ERR_INVALID, "state size mismatch");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(c->host_stage.data(), c->dU[c->cur], c->ncell * 5 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    soa_to_aos(c, c->host_stage.data(), aos);
+    return TS_OK;
+}
+
+int ts_set_time(ts_ctx* c, double time, long long iteration) {
+    if (!c) return TS_ERR_INVALID;
+    c->h_time = time;
+    c->h_iter = iteration;
+    return TS_OK;
+}
+
+int ts_enable_residual_norms(ts_ctx* c, int enable) {
+    if (!c) return TS_ERR_INVALID;
+    if (enable) return ctx_fail(c, TS_ERR_UNSUPPORTED, "residual norms are not computed by this build");
+    return TS_OK;
+}
+
+int ts_enable_dt_log(ts_ctx* c, long long capacity) {
+    if (!c || capacity < 0) return TS_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->dt_log) { HIPCHK(c, hipFree(c->dt_log)); c->dt_log = nullptr; }
+    c->dt_log_cap = capacity;
+    c->dt_log_base = c->h_iter;
+    if (capacity > 0) HIPCHK(c, hipMalloc(&c->dt_log, capacity * sizeof(double)));
+    return TS_OK;
+}
+
+int ts_get_dt_log(ts_ctx* c, double* out, long long capacity, long long* n_out) {
+    if (!c || !out || !n_out) return TS_ERR_INVALID;
+    long long n = std::min<long long>(c->h_iter - c->dt_log_base, c->dt_log_cap);
+    n = std::min(n, capacity);
+    if (n < 0) n = 0;
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(out, c->dt_log, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    *n_out = n;
+    return TS_OK;
+}
+
+int ts_enable_kernel_timing(ts_ctx* c, int enable) {
+    if (!c) return TS_ERR_INVALID;
+    c->timing = enable != 0;
+    for (int k = 0; k < 5; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    return TS_OK;
+}
+
+int ts_get_kernel_timing(ts_ctx* c, int kind, double* total_ms, long long* launches) {
+    if (!c || kind < 0 || kind > 4 || !total_ms || !launches) return TS_ERR_INVALID;
+    if (kind == 4) {
+        double s = 0;
+        long long n = 0;
+        for (int k = 0; k < 4; k++) { s += c->t_ms[k]; n += c->t_n[k]; }
+        *total_ms = s;
+        *launches = n;
+    } else {
+        *total_ms = c->t_ms[kind];
+        *launches = c->t_n[kind];
+    }
+    return TS_OK;
+}
+
+int ts_memory_footprint(const ts_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return TS_ERR_INVALID;
+    *bytes = c->device_bytes;
+    return TS_OK;
+}
+
+int ts_algorithmic_bytes_per_cell(const ts_ctx* c, int kind, double* bytes) {
+    if (!c || !bytes || kind < 0 || kind > 4) return TS_ERR_INVALID;
+    // one state array = 5 * sizeof(real) per cell; halo re-reads are not counted
+    const double d = 40.0;
+    const double s = c->cfg.precision == TS_FP32_RESIDUAL ? 20.0 : 40.0;
+    double b[5];
+    b[0] = d;  // dt pass reads U^n
+    if (c->cfg.rk_stages == 1) {
+        b[1] = 2 * d;  // read U^n, write U^{n+1}
+        b[2] = b[3] = 0;
+    } else {
+        b[1] = d + s;        // read U^n, write U^1
+        b[2] = s + d + s;    // read U^1, U^n, write U^2
+        b[3] = s + d + d;    // read U^2, U^n, write U^{n+1}
+    }
+    b[4] = b[0] + b[1] + b[2] + b[3];
+    *bytes = b[kind];
+    return TS_OK;
+}
+
+int ts_step(ts_ctx* c, int max_steps, double t_max, double next_snapshot_time, ts_step_info* out) {
+    if (!c || max_steps < 0) return TS_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = run_steps(c, max_steps, t_max, next_snapshot_time);
+    if (out) {
+        out->time = c->h_time;
+        out->dt = c->h_dt;
+        out->iteration = c->h_iter;
+        for (int v = 0; v < 5; v++) out->residual_norm[v] = 0.0;
+        out->steps_done = c->h_steps_done;
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,404 @@
+// ts_solver.cpp -- device set-up, state layout conversion and the step loop of one context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ts_internal.hpp"
+
+using namespace tsd;
+
+#define CHK(call)                                        \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) {                          \
+            c->err = tsr::hip_msg(e_, #call);            \
+            return TS_ERR_HIP;                           \
+        }                                                \
+    } while (0)
+
+namespace {
+
+template <typename T>
+int dalloc(ts_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        c->err = tsr::hip_msg(e, "hipMalloc");
+        return e == hipErrorOutOfMemory ? TS_ERR_NOMEM : TS_ERR_HIP;
+    }
+    c->dbufs.push_back((double*)q);
+    c->device_bytes += count * sizeof(T);
+    *p = (T*)q;
+    return TS_OK;
+}
+
+template <typename T>
+int upload(ts_ctx* c, const T* host, size_t count, const T** dev) {
+    T* d = nullptr;
+    int r = dalloc(c, &d, count);
+    if (r) return r;
+    CHK(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *dev = d;
+    return TS_OK;
+}
+
+inline size_t gidx(const ts_ctx* c, int i, int j, int k) {
+    size_t nxa = c->n[0] + 2, nya = c->nd > 1 ? c->n[1] + 2 : 1;
+    return ((size_t)k * nya + j) * nxa + i;
+}
+
+constexpr int CHUNK = 256;  // steps issued between host checks of the termination flag
+
+}  // namespace
+
+int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
+    CHK(hipSetDevice(cfg->device));
+    CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const ts_config& k = c->cfg;
+    GridArgs& g = c->grid;
+    g.nd = c->nd;
+    for (int a = 0; a < 3; a++) {
+        g.n[a] = c->n[a];
+        g.periodic[a] = a < c->nd ? k.periodic[a] : 1;
+    }
+    g.gamma = k.gamma;
+    g.eps = k.eps;
+    g.cfl = k.cfl;
+    bool uniform = true;
+    for (int a = 0; a < 3; a++) {
+        std::vector<double> rc, rl, rr;
+        if (a < c->nd) {
+            tsr::build_axis(*cfg, a, c->X[a], c->H[a]);
+            uniform = uniform && tsr::uniform_axis(c->H[a]);
+            int n = c->n[a];
+            rc.resize(n);
+            rl.resize(n);
+            rr.resize(n);
+            const std::vector<double>& X = c->X[a];
+            for (int i = 0; i < n; i++) {
+                rc[i] = 1.0 / (X[i + 2] - X[i]);
+                rl[i] = 1.0 / (X[i + 1] - X[i]);
+                rr[i] = 1.0 / (X[i + 2] - X[i + 1]);
+            }
+        } else {
+            c->X[a].assign(1, 0.0);
+            c->H[a].assign(1, 1.0);  // grid.h:343-344
+            rc.assign(1, 0.0);
+            rl.assign(1, 0.0);
+            rr.assign(1, 0.0);
+        }
+        int r;
+        if ((r = upload(c, c->H[a].data(), c->H[a].size(), &g.h[a]))) return r;
+        if ((r = upload(c, rc.data(), rc.size(), &g.rc[a]))) return r;
+        if ((r = upload(c, rl.data(), rl.size(), &g.rl[a]))) return r;
+        if ((r = upload(c, rr.data(), rr.size(), &g.rr[a]))) return r;
+    }
+    g.eps2 = 0.0;
+    if (k.recon == TS_RECON_LINEAR_LIM && k.limiter == TS_LIMITER_VENKAT) {
+        if (!uniform) {
+            c->err = "the Venkatakrishnan limiter is built for uniform grids only";
+            return TS_ERR_UNSUPPORTED;
+        }
+        // limVenkatar::SendCellInfo (Limiters.h:227-230) with K = 5 (Constants.h:11)
+        double hx = c->H[0][1], hy = c->nd > 1 ? c->H[1][1] : 1.0, hz = c->nd > 2 ? c->H[2][1] : 1.0;
+        double dx = std::pow(hx * hy * hz, 1.0 / c->nd);
+        g.eps2 = std::pow(5.0 * dx, 3.0);
+    }
+    for (int m = 0; m < TS_MAX_BC; m++) {
+        const ts_bc_config& b = k.bc[m];
+        g.bc[m].type = m < k.n_bc ? b.type : TS_BC_NATURAL;
+        for (int a = 0; a < 3; a++) g.bc[m].vel[a] = b.velocity[a];
+        g.bc[m].p = b.pstatic;
+        g.bc[m].rho = b.density;
+    }
+    for (int s = 0; s < 6; s++) {
+        g.side_marker[s] = k.side_marker[s];
+        g.face_marker[s] = nullptr;
+        int a = s / 2;
+        if (a < c->nd && cfg->face_marker[s]) {
+            size_t nf = c->ncell / c->n[a];
+            for (size_t f = 0; f < nf; f++)
+                if (cfg->face_marker[s][f] >= k.n_bc) {
+                    c->err = "face marker refers to a missing boundary condition";
+                    return TS_ERR_INVALID;
+                }
+            c->hmarker[s].assign(cfg->face_marker[s], cfg->face_marker[s] + nf);
+            const uint8_t* d = nullptr;
+            int r = upload(c, cfg->face_marker[s], nf, &d);
+            if (r) return r;
+            g.face_marker[s] = d;
+        }
+    }
+    if (!tsr::lookup_kernels(c->nd, k.riemann, tsr::recon_variant(k), k.precision, &c->ks) || !c->ks.dt) {
+        c->err = "no kernel instance for this (dimension, flux, reconstruction) combination";
+        return TS_ERR_UNSUPPORTED;
+    }
+    // state buffers
+    int r;
+    double* u0 = nullptr;
+    if ((r = dalloc(c, &u0, c->ncell * 5))) return r;
+    c->dU[0] = u0;
+    c->stage_elem = (k.rk_stages == 3 && k.precision == TS_FP32_RESIDUAL) ? 4 : 8;
+    int nstage = k.rk_stages == 3 ? 2 : 1;
+    for (int s = 1; s <= nstage; s++) {
+        double* p = nullptr;
+        size_t count = c->stage_elem == 8 ? c->ncell * 5 : (c->ncell * 5 + 1) / 2;
+        if ((r = dalloc(c, &p, count))) return r;
+        c->dU[s] = p;
+    }
+    if ((r = dalloc(c, &c->dctl, 1))) return r;
+    CHK(hipHostMalloc((void**)&c->hctl, sizeof(StepCtl), hipHostMallocDefault));
+    std::memset(c->hctl, 0, sizeof(StepCtl));
+    // tiling
+    if (c->nd == 1) {
+        c->ty = 1;
+        c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
+        c->ntiles = c->nstrips;
+        c->block_dim = dim3(64);
+        c->grid_dim = dim3(c->ntiles);
+    } else if (c->nd == 2) {
+        c->ty = tsr::tuned_rows_per_tile(*cfg);
+        c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
+        int nyb = (c->n[1] + c->ty - 1) / c->ty;
+        c->ntiles = c->nstrips * nyb;
+        c->block_dim = dim3(256);
+        c->grid_dim = dim3((c->ntiles + 3) / 4);
+    } else {
+        c->ty = tsr::tuned_rows_per_tile(*cfg);
+        c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
+        int nyb = (c->n[1] + c->ty - 1) / c->ty;
+        int nzb = (c->n[2] + ZPLANES - 1) / ZPLANES;
+        c->ntiles = c->nstrips * nyb * nzb;
+        c->block_dim = dim3(64 * ZPLANES);
+        c->grid_dim = dim3(c->ntiles);
+    }
+    c->host_stage.resize(c->ncell * 5);
+    return TS_OK;
+}
+
+void destroy_device_ctx(ts_ctx* c) {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; i++)
+        if (c->graph_exec[i]) hipGraphExecDestroy(c->graph_exec[i]);
+    for (double* p : c->dbufs) hipFree(p);
+    c->dbufs.clear();
+    if (c->dt_log) hipFree(c->dt_log);
+    if (c->hctl) hipHostFree(c->hctl);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+void aos_to_soa(const ts_ctx* c, const double* aos, double* soa) {
+    int gy = c->nd > 1 ? 1 : 0, gz = c->nd > 2 ? 1 : 0;
+    size_t P = c->ncell;
+    for (int k = 0; k < c->n[2]; k++)
+        for (int j = 0; j < c->n[1]; j++)
+            for (int i = 0; i < c->n[0]; i++) {
+                size_t s = gidx(c, i + 1, j + gy, k + gz) * 5;
+                size_t d = ((size_t)k * c->n[1] + j) * c->n[0] + i;
+                for (int v = 0; v < 5; v++) soa[v * P + d] = aos[s + v];
+            }
+}
+
+void soa_to_aos(const ts_ctx* c, const double* soa, double* aos) {
+    size_t total = ts_state_size(c);
+    std::fill(aos, aos + total, 0.0);
+    int g[3] = {1, c->nd > 1 ? 1 : 0, c->nd > 2 ? 1 : 0};
+    size_t P = c->ncell;
+    for (int k = 0; k < c->n[2]; k++)
+        for (int j = 0; j < c->n[1]; j++)
+            for (int i = 0; i < c->n[0]; i++) {
+                size_t d = gidx(c, i + g[0], j + g[1], k + g[2]) * 5;
+                size_t s = ((size_t)k * c->n[1] + j) * c->n[0] + i;
+                for (int v = 0; v < 5; v++) aos[d + v] = soa[v * P + s];
+            }
+    // ghost layers: periodic images or boundary-condition images (kernel.h:334-915)
+    for (int a = 0; a < c->nd; a++) {
+        int ja = a == 0 ? 1 : 0, ka = a == 2 ? 1 : 2;
+        int nj = c->n[ja], nk = c->n[ka];
+        for (int kk = 0; kk < nk; kk++)
+            for (int jj = 0; jj < nj; jj++)
+                for (int side = 0; side < 2; side++) {
+                    int I[3], S[3];
+                    I[ja] = S[ja] = jj + g[ja];
+                    I[ka] = S[ka] = kk + g[ka];
+                    int n = c->n[a];
+                    if (c->cfg.periodic[a]) {
+                        I[a] = side ? n + 1 : 0;
+                        S[a] = side ? 1 : n;
+                        for (int v = 0; v < 5; v++) aos[gidx(c, I[0], I[1], I[2]) * 5 + v] = aos[gidx(c, S[0], S[1], S[2]) * 5 + v];
+                    } else {
+                        I[a] = side ? n + 1 : 0;
+                        S[a] = side ? n : 1;
+                        int s = 2 * a + side;
+                        int face = a == 0 ? kk * c->n[1] + jj : kk * c->n[0] + jj;
+                        int m = c->hmarker[s].empty() ? c->cfg.side_marker[s] : c->hmarker[s][face];
+                        tsr::host_bc_image(c->cfg.bc[m], c->cfg.gamma, a, &aos[gidx(c, S[0], S[1], S[2]) * 5],
+                                           &aos[gidx(c, I[0], I[1], I[2]) * 5]);
+                    }
+                }
+    }
+}
+
+namespace {
+
+struct Launch {
+    LaunchFn fn;
+    KArgs args;
+    int kind;
+};
+
+// the launches of step s of a chunk, given the buffer holding U^n at the chunk start
+void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[4], int& n) {
+    KArgs a{};
+    a.g = c->grid;
+    a.plane = c->ncell;
+    a.ctl = c->dctl;
+    a.ty = c->ty;
+    a.nstrips = c->nstrips;
+    a.ntiles = c->ntiles;
+    a.un = (const double*)c->dU[0];
+    n = 0;
+    if (c->cfg.rk_stages == 1) {
+        int in = (start_buf + s) & 1, out = in ^ 1;
+        a.uin = c->dU[in];
+        a.uout = nullptr;
+        L[n++] = {c->ks.dt, a, 0};
+        a.uout = c->dU[out];
+        L[n++] = {c->ks.s1e, a, 1};
+    } else {
+        a.uin = c->dU[0];
+        L[n++] = {c->ks.dt, a, 0};
+        a.uout = c->dU[1];
+        L[n++] = {c->ks.s1r, a, 1};
+        a.uin = c->dU[1];
+        a.uout = c->dU[2];
+        L[n++] = {c->ks.s2, a, 2};
+        a.uin = c->dU[2];
+        a.uout = c->dU[0];
+        L[n++] = {c->ks.s3, a, 3};
+    }
+}
+
+int issue_direct(ts_ctx* c, int start_buf, int first, int count) {
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> rec;
+    size_t need = c->timing ? (size_t)count * 8 : 0;
+    while (c->events.size() < need) {
+        hipEvent_t e;
+        CHK(hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    size_t ev = 0;
+    for (int s = first; s < first + count; s++) {
+        Launch L[4];
+        int n;
+        step_launches(c, start_buf, s, L, n);
+        for (int i = 0; i < n; i++) {
+            if (c->timing) {
+                hipEvent_t e0 = c->events[ev++], e1 = c->events[ev++];
+                CHK(hipEventRecord(e0, c->stream));
+                L[i].fn(L[i].args, c->grid_dim, c->block_dim, c->stream);
+                CHK(hipEventRecord(e1, c->stream));
+                rec.push_back({L[i].kind, {e0, e1}});
+            } else {
+                L[i].fn(L[i].args, c->grid_dim, c->block_dim, c->stream);
+            }
+        }
+    }
+    CHK(hipGetLastError());
+    if (c->timing) {
+        CHK(hipStreamSynchronize(c->stream));
+        for (auto& r : rec) {
+            float ms = 0;
+            CHK(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+            c->t_ms[r.first] += ms;
+            c->t_n[r.first] += 1;
+        }
+    }
+    return TS_OK;
+}
+
+int issue_graph(ts_ctx* c, int start_buf, int count) {
+    int gi = start_buf & 1;
+    if (!c->graph_exec[gi] || c->graph_steps[gi] != count) {
+        if (c->graph_exec[gi]) {
+            CHK(hipGraphExecDestroy(c->graph_exec[gi]));
+            c->graph_exec[gi] = nullptr;
+        }
+        hipGraph_t graph;
+        CHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < count; s++) {
+            Launch L[4];
+            int n;
+            step_launches(c, start_buf, s, L, n);
+            for (int i = 0; i < n; i++) L[i].fn(L[i].args, c->grid_dim, c->block_dim, c->stream);
+        }
+        CHK(hipStreamEndCapture(c->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&c->graph_exec[gi], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            c->err = tsr::hip_msg(e, "hipGraphInstantiate");
+            c->graph_exec[gi] = nullptr;
+            return TS_ERR_HIP;
+        }
+        c->graph_steps[gi] = count;
+    }
+    CHK(hipGraphLaunch(c->graph_exec[gi], c->stream));
+    return TS_OK;
+}
+
+}  // namespace
+
+int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap) {
+    StepCtl* h = c->hctl;
+    std::memset(h, 0, sizeof(StepCtl));
+    h->time = c->h_time;
+    h->dt = c->h_dt;
+    h->t_max = t_max;
+    h->next_snap = next_snap;
+    h->iter = c->h_iter;
+    h->max_steps = max_steps;
+    h->steps_done = 0;
+    h->active = max_steps > 0 ? 1 : 0;
+    h->go = 0;
+    h->error = 0;
+    h->min_bits = ~0ull;
+    h->blocks_done = 0;
+    h->dt_log = c->dt_log;
+    h->dt_log_base = c->dt_log_base;
+    h->dt_log_cap = c->dt_log ? c->dt_log_cap : 0;
+    CHK(hipMemcpyAsync(c->dctl, h, sizeof(StepCtl), hipMemcpyHostToDevice, c->stream));
+    int launched = 0;
+    const int start_buf = c->cur;
+    int rc = TS_OK;
+    while (launched < max_steps) {
+        int chunk = std::min(max_steps - launched, CHUNK);
+        // executed steps always form a prefix of the issued ones, so the Euler ping-pong buffer
+        // of the final state is (start + steps_done) mod 2 whatever the chunking
+        int sb = (start_buf + launched) & 1;
+        if (c->use_graphs && !c->timing && chunk == CHUNK) rc = issue_graph(c, sb, chunk);
+        else rc = issue_direct(c, sb, 0, chunk);
+        if (rc) return rc;
+        launched += chunk;
+        CHK(hipMemcpyAsync(h, c->dctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+        CHK(hipStreamSynchronize(c->stream));
+        if (!h->active) break;
+    }
+    if (launched == 0) {
+        CHK(hipMemcpyAsync(h, c->dctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+        CHK(hipStreamSynchronize(c->stream));
+    }
+    c->h_time = h->time;
+    c->h_dt = h->dt;
+    c->h_iter = h->iter;
+    c->h_steps_done = h->steps_done;
+    if (c->cfg.rk_stages == 1) c->cur = (start_buf + h->steps_done) & 1;
+    if (h->error) {
+        c->err = "dt is not finite and positive (non-physical state)";
+        return TS_ERR_UNPHYSICAL;
+    }
+    return TS_OK;
+}
+

@@ -1,0 +1,170 @@
+"""Pins of the oracle's Riemann solvers (CPU).  Each check is chosen so that a dropped term, a
+wrong sign or a swapped operand fails: consistency, upwinding, exact resolution of stationary
+discontinuities, rotation, and the star-state table the reference itself carries."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from turbostructred_amd import capi
+import util
+
+G = 1.4
+HERE = os.path.dirname(os.path.abspath(__file__))
+FLUXES = [capi.RIEMANN_ROE, capi.RIEMANN_HLLC, capi.RIEMANN_GODUNOV]
+E = np.eye(3)
+
+
+def cons(r, u, v, w, p, g=G):
+    return util.prim2cons(r, u, v, w, p, g)
+
+
+def phys(U, n, g=G):
+    r = U[0]
+    vel = U[1:4] / r
+    p = (g - 1) * (U[4] - 0.5 * r * vel @ vel)
+    vn = vel @ n
+    return np.array([r * vn, *(r * vn * vel + p * np.asarray(n)), vn * (U[4] + p)])
+
+
+def rng_states(k, seed=1):
+    r = np.random.default_rng(seed)
+    for _ in range(k):
+        yield cons(r.uniform(0.1, 2), r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(0.1, 3))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_consistency(flux):
+    for U in rng_states(50):
+        for a in range(3):
+            F, lam = po.flux(flux, G, U, U, E[a])
+            np.testing.assert_allclose(F, phys(U, E[a]), rtol=1e-13, atol=1e-13)
+            c = np.sqrt(G * ((G - 1) * (U[4] - 0.5 * (U[1:4] @ U[1:4]) / U[0])) / U[0])
+            un = U[1 + a] / U[0]
+            assert lam == pytest.approx(abs(un) + c, rel=1e-12)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_supersonic_upwinding(flux):
+    # every wave moves right: the flux is the left physical flux, and the mirror case
+    UL = cons(1.0, 3.0, 0.2, -0.1, 1.0)
+    UR = cons(0.5, 2.8, -0.3, 0.4, 0.6)
+    F, _ = po.flux(flux, G, UL, UR, E[0])
+    np.testing.assert_allclose(F, phys(UL, E[0]), rtol=1e-12)
+    UL2 = cons(0.5, -2.8, 0.1, 0.0, 0.6)
+    UR2 = cons(1.0, -3.0, 0.3, 0.2, 1.0)
+    F, _ = po.flux(flux, G, UL2, UR2, E[0])
+    np.testing.assert_allclose(F, phys(UR2, E[0]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_stationary_contact_resolved(flux):
+    # u = 0, equal pressure, density jump: the exact interface flux is (0, p, 0, 0, 0)
+    UL = cons(1.0, 0.0, 0.3, 0.0, 0.7)
+    UR = cons(0.2, 0.0, -0.4, 0.0, 0.7)
+    F, _ = po.flux(flux, G, UL, UR, E[0])
+    np.testing.assert_allclose(F, [0, 0.7, 0, 0, 0], atol=1e-14)
+
+
+@pytest.mark.parametrize("flux", [capi.RIEMANN_ROE, capi.RIEMANN_GODUNOV])
+def test_stationary_shock_resolved(flux):
+    # Rankine-Hugoniot: a normal shock standing at the face (Mach 2 upstream)
+    M, rl, pl = 2.0, 1.0, 1.0
+    cl = np.sqrt(G * pl / rl)
+    ul = M * cl
+    rr = rl * (G + 1) * M**2 / ((G - 1) * M**2 + 2)
+    ur = rl * ul / rr
+    pr = pl * (2 * G * M**2 - (G - 1)) / (G + 1)
+    UL, UR = cons(rl, ul, 0, 0, pl), cons(rr, ur, 0, 0, pr)
+    np.testing.assert_allclose(phys(UL, E[0]), phys(UR, E[0]), rtol=1e-13)
+    F, _ = po.flux(flux, G, UL, UR, E[0])
+    np.testing.assert_allclose(F, phys(UL, E[0]), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_rotation_invariance(flux):
+    perm = {1: [0, 2, 1, 3, 4], 2: [0, 3, 2, 1, 4]}  # swap x with y / z momentum
+    for UL, UR in zip(rng_states(20, 3), rng_states(20, 4)):
+        Fx, lx = po.flux(flux, G, UL, UR, E[0])
+        for a in (1, 2):
+            p = perm[a]
+            Fa, la = po.flux(flux, G, UL[p], UR[p], E[a])
+            np.testing.assert_allclose(Fa[p], Fx, rtol=1e-13, atol=1e-13)
+            assert la == pytest.approx(lx, rel=1e-14)
+
+
+def test_roe_harten_fix_only_near_zero_eigenvalue():
+    UL, UR = cons(1.0, 0.1, 0, 0, 1.0), cons(0.9, 0.12, 0, 0, 0.95)
+    F0, _ = po.flux(capi.RIEMANN_ROE, G, UL, UR, E[0], eps=0.0)
+    Fs, _ = po.flux(capi.RIEMANN_ROE, G, UL, UR, E[0], eps=1e-6)
+    np.testing.assert_array_equal(F0, Fs)  # |u| = 0.1 > eps * lambda: no change
+    Fb, _ = po.flux(capi.RIEMANN_ROE, G, UL, UR, E[0], eps=0.5)
+    assert np.abs(Fb - F0).max() > 1e-6  # |u| < eps * lambda: entropy fix active
+
+
+def test_sod_first_dt_closed_form():
+    # faces left of the diaphragm carry |u|+c = sqrt(1.4), so dt0 = 0.4 h / (2 sqrt 1.4)
+    UL = cons(1, 0, 0, 0, 1)
+    _, lam = po.flux(capi.RIEMANN_ROE, G, UL, UL, E[0])
+    assert lam == np.sqrt(1.4)
+
+
+def test_toro_star_table():
+    tab = json.load(open(os.path.join(HERE, "golden", "toro_star_table.json")))
+    for name, t in tab["tests"].items():
+        s = po.star_state(tab["gamma"], *t["left"], *t["right"])
+        for key, got in (("pstar", s["p"]), ("ustar", s["u"]), ("rhostarL", s["rl"]), ("rhostarR", s["rr"])):
+            want = t[key]
+            # table holds 5-6 significant digits
+            assert abs(got - want) <= 5e-5 * max(abs(want), 1e-2) + 1e-5, (name, key, got, want)
+        assert not s["vacuum"]
+
+
+def test_star_pressure_is_a_root():
+    r = np.random.default_rng(7)
+    for _ in range(200):
+        rl, rr = r.uniform(0.1, 2, 2)
+        ul, ur = r.uniform(-1, 1, 2)
+        pl, pr = r.uniform(0.1, 10, 2)
+        g = r.choice([1.4, 1.67])
+        s = po.star_state(g, rl, ul, pl, rr, ur, pr)
+        if s["vacuum"]:
+            continue
+
+        def fk(p, rk, pk):
+            ak = np.sqrt(g * pk / rk)
+            if p > pk:
+                A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+                return (p - pk) * np.sqrt(A / (p + B))
+            return 2 * ak / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+
+        f = fk(s["p"], rl, pl) + fk(s["p"], rr, pr) + (ur - ul)
+        assert abs(f) < 1e-12 * (abs(ul) + abs(ur) + np.sqrt(g * pl / rl) + np.sqrt(g * pr / rr))
+
+
+def test_vacuum_generation():
+    # u_R - u_L >= 2 (aL + aR)/(g - 1): the star pressure is 0 and the middle is empty
+    s = po.star_state(1.4, 1.0, -4.0, 0.4, 1.0, 4.0, 0.4)
+    assert s["vacuum"] and s["p"] == 0.0
+    assert np.all(po.exact_sample(1.4, 1.0, -4.0, 0.4, 1.0, 4.0, 0.4, 0.0) == 0.0)
+    F, _ = po.flux(capi.RIEMANN_GODUNOV, 1.4, cons(1, -4, 0, 0, 0.4, 1.4), cons(1, 4, 0, 0, 0.4, 1.4), E[0])
+    np.testing.assert_array_equal(F, 0.0)
+
+
+def test_exact_sod_profile_landmarks():
+    # Sod at t = 0.25, x0 = 0.5 (the reference's ToroTests case 1)
+    t, g = 0.25, 1.4
+    cl = np.sqrt(g)
+    head = 0.5 - cl * t  # rarefaction head
+    samp = lambda x: po.exact_sample(g, 1, 0, 1, 0.125, 0, 0.1, (x - 0.5) / t)
+    np.testing.assert_allclose(samp(head - 1e-9), [1, 0, 1], atol=1e-7)
+    s = po.star_state(g, 1, 0, 1, 0.125, 0, 0.1)
+    xc = 0.5 + s["u"] * t  # contact
+    np.testing.assert_allclose(samp(xc - 1e-6), [s["rl"], s["u"], s["p"]], rtol=1e-9)
+    np.testing.assert_allclose(samp(xc + 1e-6), [s["rr"], s["u"], s["p"]], rtol=1e-9)
+    # inside the fan the isentrope p/rho^g is constant and u + 2c/(g-1) is the left invariant
+    rho, u, p = samp(0.4)
+    assert p / rho**g == pytest.approx(1.0, rel=1e-12)
+    assert u + 2 * np.sqrt(g * p / rho) / (g - 1) == pytest.approx(2 * cl / (g - 1), rel=1e-12)

@@ -1,0 +1,196 @@
+"""Pins of the oracle's reconstruction, boundary images and whole step (CPU).
+
+Reconstruction: exactness on linear data, the ENO smallest-|slope| rule without a sign check
+(ENO2PointsStencil.h:211), limiter bounds and closed forms.  Whole step: the Sod conservation
+invariants (mass and energy constant, momentum grows by exactly (pL - pR) dt per step while the
+waves are inside), first-order convergence to the exact solution, x/y/z direction invariance,
+periodic conservation and the diagonal symmetry of Lax-Liu configuration 3."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from turbostructred_amd import capi
+import util
+
+G = 1.4
+
+
+def test_pwc_and_linear_exactness():
+    h = [0.1, 0.2]
+    c = np.array([1.0, 0.5, -0.2, 0.0, 2.5])
+    sx = np.array([0.3, -0.1, 0.05, 0.0, 0.2])
+    sy = np.array([-0.2, 0.4, 0.1, 0.0, -0.3])
+    nb = np.stack([c - sx * h[0], c + sx * h[0], c - sy * h[1], c + sy * h[1]])
+    g, lim = po.reconstruct(capi.RECON_PWC, 0, 2, h, c, nb)
+    assert np.all(g == 0) and np.all(lim == 1)
+    for rc in (capi.RECON_LINEAR, capi.RECON_ENO2):
+        g, lim = po.reconstruct(rc, 0, 2, h, c, nb)
+        np.testing.assert_allclose(g[:, 0], sx, atol=1e-14)
+        np.testing.assert_allclose(g[:, 1], sy, atol=1e-14)
+
+
+def test_eno_picks_smaller_slope_without_sign_check():
+    h = [1.0]
+    c = np.array([1.0, 0, 0, 0, 2.0])
+    nb = np.stack([c - np.array([-1.0, 0, 0, 0, 0.5]), c + np.array([2.0, 0, 0, 0, -3.0])])
+    g, _ = po.reconstruct(capi.RECON_ENO2, 0, 1, h, c, nb)
+    assert g[0, 0] == -1.0  # |left| = 1 < |right| = 2: left slope, opposite sign kept
+    assert g[4, 0] == 0.5
+
+
+@pytest.mark.parametrize("lim", [capi.LIMITER_BJ, capi.LIMITER_VENKAT])
+def test_limiter_bounds(lim):
+    r = np.random.default_rng(5)
+    for _ in range(300):
+        c = r.uniform(0.5, 2, 5)
+        nb = c + r.normal(0, 0.3, (6, 5))
+        g, l = po.reconstruct(capi.RECON_LINEAR_LIM, lim, 3, [0.1, 0.1, 0.1], c, nb)
+        assert np.all(l >= 0) and np.all(l <= 1 + 1e-15)
+        # limited face values stay inside the neighbour range for Barth-Jespersen
+        if lim == capi.LIMITER_BJ:
+            for a in range(3):
+                for s in (-1, 1):
+                    fv = c + l * g[:, a] * (0.5 * 0.1 * s)
+                    lo = np.minimum(c, nb.min(axis=0))
+                    hi = np.maximum(c, nb.max(axis=0))
+                    assert np.all(fv >= lo - 1e-14) and np.all(fv <= hi + 1e-14)
+
+
+def test_bj_closed_forms():
+    h = [1.0]
+    # monotone, smooth: projections within the bounds -> phi = 1
+    c = np.full(5, 1.0)
+    nb = np.stack([c - 0.1, c + 0.1])
+    _, l = po.reconstruct(capi.RECON_LINEAR_LIM, capi.LIMITER_BJ, 1, h, c, nb)
+    assert np.all(l == 1.0)
+    # local maximum: phi = 0
+    nb = np.stack([c - 0.1, c - 0.3])
+    _, l = po.reconstruct(capi.RECON_LINEAR_LIM, capi.LIMITER_BJ, 1, h, c, nb)
+    assert np.all(l == 0.0)
+    # steep one side: slope (u+ - u-)/2h = 0.6, plus-face projection 0.3, room to the max 0.1
+    # -> phi = 1/3 (the minus face has room 1.1 for -0.3 and does not bind)
+    nb = np.stack([c - 1.1, c + 0.1])
+    _, l = po.reconstruct(capi.RECON_LINEAR_LIM, capi.LIMITER_BJ, 1, h, c, nb)
+    np.testing.assert_allclose(l, 0.1 / 0.3, rtol=1e-14)
+
+
+def test_venkat_closed_form_at_extremum():
+    # c is the maximum and d2 > 0: dm = 0, so phi = eps2 / (2 dn^2 + eps2) with dn = d2 / un
+    h = [0.5]
+    c = np.array([1.0, 1.0, 1.0, 1.0, 1.0])
+    nb = np.stack([c - 0.4, c - 0.2])  # slope (-0.2 + 0.4)/(2*0.5) = 0.2 ... c is the max
+    g, l = po.reconstruct(capi.RECON_LINEAR_LIM, capi.LIMITER_VENKAT, 1, h, c, nb)
+    eps2 = (5 * 0.5) ** 3
+    un = 0.5 * 0.4
+    d2 = 0.2 * 0.25  # g * 0.5 h on the plus face
+    # minus face: d2 < 0, um = umin: dm = -0.4/un
+    dmm, dnm = -0.4 / un, -d2 / un
+    lm = ((dmm**2 + eps2) + 2 * dnm * dmm) / (dmm**2 + 2 * dnm**2 + dmm * dnm + eps2)
+    lp = eps2 / (2 * (d2 / un) ** 2 + eps2)
+    np.testing.assert_allclose(l, min(1.0, lm, lp), rtol=1e-13)
+
+
+def test_boundary_images():
+    U = np.array([1.2, 0.3, -0.4, 0.5, 3.0])
+    n = np.array([-1.0, 0, 0])
+    mk = lambda t, **kw: capi.make_config(bcs=[dict(type=t, **kw)]).bc[0]
+    np.testing.assert_array_equal(po.bc_image(mk(capi.BC_NATURAL), G, U, n), U)
+    np.testing.assert_array_equal(po.bc_image(mk(capi.BC_SUBSONIC_OUTLET), G, U, n), U)
+    np.testing.assert_array_equal(po.bc_image(mk(capi.BC_SYMMETRY), G, U, n), [1.2, -0.3, -0.4, 0.5, 3.0])
+    np.testing.assert_array_equal(po.bc_image(mk(capi.BC_SYMMETRY), G, U, np.array([0, 1.0, 0])), [1.2, 0.3, 0.4, 0.5, 3.0])
+    np.testing.assert_array_equal(po.bc_image(mk(capi.BC_WALL), G, U, n), [1.2, -0.3, 0.4, -0.5, 3.0])
+    vb = np.array([0.1, 0.2, 0.0])
+    mw = po.bc_image(mk(capi.BC_MOVING_WALL, velocity=tuple(vb)), G, U, n)
+    vin = U[1:4] / U[0]
+    np.testing.assert_allclose((mw[1:4] / mw[0] + vin) / 2, vb, atol=1e-15)  # wall velocity is the mean
+    ein = U[4] - 0.5 * U[0] * vin @ vin
+    np.testing.assert_allclose(mw[4] - 0.5 * mw[0] * (mw[1:4] / mw[0]) @ (mw[1:4] / mw[0]), ein, rtol=1e-14)
+    si = po.bc_image(mk(capi.BC_SUBSONIC_INLET, velocity=(0.5, 0, 0), pstatic=0.8, density=1.1), G, U, n)
+    np.testing.assert_allclose(si, util.prim2cons(1.1, 0.5, 0, 0, 0.8, G), rtol=1e-15)
+
+
+def sod_oracle(n, steps=None, t_max=0.2):
+    cfg = capi.make_config(ndims=1, nx=n, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC)
+    o = po.Oracle(cfg)
+    o.set_state(util.sod(cfg))
+    return cfg, o
+
+
+def test_sod_conservation_invariants():
+    cfg, o = sod_oracle(1000)
+    h = 1e-3
+    t = 0.0
+    for _ in range(60):
+        info = o.step(50, t_max=0.2)
+        S = capi.interior(cfg, o.get_state())[0, 0]
+        assert S[:, 0].sum() * h == pytest.approx(0.5625, rel=1e-13)
+        assert S[:, 4].sum() * h == pytest.approx(1.375, rel=1e-13)
+        # boundary faces carry the undisturbed u = 0 flux exactly: d(sum rho u h)/dt = pL - pR
+        assert S[:, 1].sum() * h == pytest.approx(0.9 * info.time, rel=1e-12)
+        if info.time >= 0.2:
+            break
+    assert info.time >= 0.2
+
+
+def l1_density_error(n):
+    cfg, o = sod_oracle(n)
+    info = o.step(10**6, t_max=0.2, next_snapshot=0.2)
+    S = capi.interior(cfg, o.get_state())[0, 0]
+    x = (np.arange(n) + 0.5) / n
+    ex = np.array([po.exact_sample(G, 1, 0, 1, 0.125, 0, 0.1, (xi - 0.5) / 0.2)[0] for xi in x])
+    return np.abs(S[:, 0] - ex).mean()
+
+
+def test_sod_converges_first_order():
+    e1, e2, e3 = l1_density_error(200), l1_density_error(400), l1_density_error(800)
+    assert e1 > e2 > e3
+    assert 0.5 < np.log2(e1 / e2) < 1.2 and 0.5 < np.log2(e2 / e3) < 1.2
+
+
+@pytest.mark.parametrize("flux", [capi.RIEMANN_ROE, capi.RIEMANN_HLLC, capi.RIEMANN_GODUNOV])
+def test_direction_invariance(flux):
+    """the same 1-D problem along x, y and z of a 3-D grid (the reference's xyzTest idea)"""
+    n = 40
+    out = []
+    for axis in range(3):
+        dims = [3, 3, 3]
+        dims[axis] = n
+        L = [0.3, 0.3, 0.3]
+        L[axis] = 1.0
+        cfg = capi.make_config(ndims=3, nx=dims[0], ny=dims[1], nz=dims[2], lx=L[0], ly=L[1], lz=L[2], cfl=0.4,
+                               riemann=flux, recon=capi.RECON_LINEAR_LIM, rk_stages=3, periodic=(1, 1, 1),
+                               bcs=[dict(type=capi.BC_NATURAL)])
+        cfg.periodic[axis] = 0
+        o = po.Oracle(cfg)
+        o.set_state(util.sod(cfg, axis=axis))
+        o.step(15)
+        S = capi.interior(cfg, o.get_state())
+        line = np.moveaxis(S, 2 - axis, 0)[:, 1, 1, :]
+        dens, mom = line[:, 0], line[:, 1 + axis]
+        out.append(np.concatenate([dens, mom, line[:, 4]]))
+    np.testing.assert_allclose(out[1], out[0], rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(out[2], out[0], rtol=1e-13, atol=1e-14)
+
+
+def test_periodic_conservation():
+    cfg = capi.make_config(ndims=2, nx=40, ny=30, cfl=0.45, periodic=(1, 1, 0), riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3)
+    U = util.quadrants(cfg)
+    o = po.Oracle(cfg)
+    o.set_state(U)
+    s0 = capi.interior(cfg, U).reshape(-1, 5).sum(axis=0)
+    o.step(30)
+    s1 = capi.interior(cfg, o.get_state()).reshape(-1, 5).sum(axis=0)
+    np.testing.assert_allclose(s1, s0, rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("flux", [capi.RIEMANN_ROE, capi.RIEMANN_HLLC])
+def test_lax_liu_diagonal_symmetry(flux):
+    cfg = capi.make_config(ndims=2, nx=48, ny=48, cfl=0.45, riemann=flux, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=3)
+    o = po.Oracle(cfg)
+    o.set_state(util.quadrants(cfg))
+    o.step(20)
+    S = capi.interior(cfg, o.get_state())[0]
+    np.testing.assert_allclose(S[..., 0], S[..., 0].T, rtol=1e-12)
+    np.testing.assert_allclose(S[..., 1], S[..., 2].T, rtol=1e-12, atol=1e-12)

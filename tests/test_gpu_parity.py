@@ -207,3 +207,56 @@ def test_unphysical_state_reports_error():
     with pytest.raises(capi.TSError) as e:
         gpu.step(5)
     assert e.value.code == capi.TS_ERR_UNPHYSICAL
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("precision", [capi.FP64])
+def test_rk3_separate_dt_pass_path(flux, precision, monkeypatch):
+    """SSP-RK3 with the stand-alone dt pass (the fused-dt path is the default)."""
+    monkeypatch.setenv("TS_RK3_SEPARATE_DT", "1")
+    cfg = capi.make_config(ndims=2, nx=77, ny=64, cfl=0.45, riemann=flux, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=3, precision=precision)
+    U = util.quadrants(cfg)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 20)
+    check(a, b, dg, dr, gi, ri)
+
+
+def tgv(cfg, seed=3, amp=1e-3):
+    """compressible Taylor-Green vortex on [0, 2pi]^3 (M = 0.1), seeded density perturbation"""
+    g = cfg.gamma
+    rng = np.random.default_rng(seed)
+
+    def f(x, y, z):
+        p0 = 1.0 / (g * 0.1**2)
+        u = np.sin(x) * np.cos(y) * np.cos(z)
+        v = -np.cos(x) * np.sin(y) * np.cos(z)
+        p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16
+        rho = 1 + amp * rng.uniform(-1, 1, x.shape)
+        return util.prim2cons(rho, u, v, 0 * x, p, g)
+
+    return util.fill_state(cfg, f)
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_3d_matrix(recon, lim, flux, stages):
+    n = (37, 21, 17)
+    cfg = capi.make_config(ndims=3, nx=n[0], ny=n[1], nz=n[2], lx=2 * np.pi, ly=2 * np.pi * 21 / 37,
+                           lz=2 * np.pi * 17 / 37, cfl=0.45, riemann=flux, recon=recon, limiter=lim,
+                           rk_stages=stages, periodic=(1, 0, 0),
+                           bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL), dict(type=capi.BC_SYMMETRY)],
+                           side_marker=(0, 0, 1, 0, 2, 0))
+    U = tgv(cfg)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 12)
+    check(a, b, dg, dr, gi, ri)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 2), (5, 7, 3), (62, 5, 13), (31, 17, 7)])
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0)])
+def test_3d_edge_sizes(shape, periodic):
+    cfg = capi.make_config(ndims=3, nx=shape[0], ny=shape[1], nz=shape[2], cfl=0.4, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3, periodic=periodic)
+    U = tgv(cfg, amp=0.05)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 8)
+    check(a, b, dg, dr, gi, ri)

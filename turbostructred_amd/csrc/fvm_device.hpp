@@ -83,7 +83,21 @@ __device__ inline double fdiv(double a, double b) {
     return __builtin_fma(e, r, q);
 }
 __device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ inline double fsqrt(double x) { return __builtin_sqrt(x); }
+// sqrt from the reciprocal-square-root estimate with two Newton-Raphson corrections (no
+// rescaling: for normal-range arguments; 0 maps to 0, negative arguments to NaN)
+__device__ inline double fsqrt(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    double y = x * r;
+    double h = 0.5 * r;
+    double e = __builtin_fma(-h, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-y, y, x);
+    y = __builtin_fma(d, h, y);
+    d = __builtin_fma(-y, y, x);
+    y = __builtin_fma(d, h, y);
+    return x == 0.0 ? 0.0 : y;
+}
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename R> __device__ inline R tiny_spread();
@@ -137,44 +151,60 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                 R umax = c[v], umin = c[v];
 #pragma unroll
                 for (int n = 0; n < 2 * ND; n++) {
-                    umax = (umax < nb[n][v]) ? nb[n][v] : umax;
-                    umin = (umin > nb[n][v]) ? nb[n][v] : umin;
+                    umax = __builtin_fmax(umax, nb[n][v]);
+                    umin = __builtin_fmin(umin, nb[n][v]);
                 }
                 if (!(umax - umin < tiny_spread<R>())) {
-                    R bestA = R(1), bestD = R(1);
-                    R inv_un = R(0);
-                    if constexpr (LIM == LIM_VENKAT) inv_un = frcp(R(0.5) * (umax - umin));
+                    if constexpr (LIM == LIM_BJ) {
+                        // Barth-Jespersen (Limiters.h:214-216).  Along each axis the two face
+                        // projections are +-d_a, one bounded by umax and the other by umin, so
+                        // min over faces of (u_m - u_i)/d2 = min(umax - c, c - umin) / max_a |d_a|.
+                        R m = __builtin_fmin(umax - c[v], c[v] - umin);
+                        R dmax = R(0);
 #pragma unroll
-                    for (int a = 0; a < ND; a++) {
-#pragma unroll
-                        for (int s = 0; s < 2; s++) {
-                            R pt = s ? geo.hh[a] : -geo.hh[a];
+                        for (int a = 0; a < ND; a++) {
                             R d2;
                             {
 #pragma clang fp contract(off)
                                 // projection (v + g.p) - v, rounded as Linear2psLim.h:77-89
-                                R fv = c[v] + g[a] * pt;
+                                R fv = c[v] + g[a] * geo.hh[a];
                                 d2 = fv - c[v];
                             }
-                            if (d2 > R(0) || d2 < R(0)) {
-                                R um = (d2 > R(0)) ? umax : umin;
-                                R A, D;
-                                if constexpr (LIM == LIM_BJ) {  // Limiters.h:214-216
-                                    A = __builtin_fabs(um - c[v]);
-                                    D = __builtin_fabs(d2);
-                                } else {  // Limiters.h:236-249
+                            dmax = __builtin_fmax(dmax, __builtin_fabs(d2));
+                        }
+                        if (m < dmax) {
+                            // the divisor can be subnormal (a variable that is zero up to
+                            // round-off): the reciprocal estimate is only used above that range
+                            lim = (dmax > R(1e-30)) ? fdiv(m, dmax) : m / dmax;
+                        }
+                    } else {
+                        R bestA = R(1), bestD = R(1);
+                        R inv_un = frcp(R(0.5) * (umax - umin));
+#pragma unroll
+                        for (int a = 0; a < ND; a++) {
+#pragma unroll
+                            for (int s = 0; s < 2; s++) {
+                                R pt = s ? geo.hh[a] : -geo.hh[a];
+                                R d2;
+                                {
+#pragma clang fp contract(off)
+                                    R fv = c[v] + g[a] * pt;
+                                    d2 = fv - c[v];
+                                }
+                                if (d2 > R(0) || d2 < R(0)) {
+                                    // Venkatakrishnan (Limiters.h:236-249), normalised by
+                                    // (umax - umin)/2
+                                    R um = (d2 > R(0)) ? umax : umin;
                                     R dm = um * inv_un - c[v] * inv_un;
                                     R dn = d2 * inv_un;
-                                    A = (dm * dm + geo.eps2) + R(2) * dn * dm;
-                                    D = dm * dm + R(2) * dn * dn + dm * dn + geo.eps2;
+                                    R A = (dm * dm + geo.eps2) + R(2) * dn * dm;
+                                    R D = dm * dm + R(2) * dn * dn + dm * dn + geo.eps2;
+                                    if (A * bestD < bestA * D) { bestA = A; bestD = D; }
                                 }
-                                if (A * bestD < bestA * D) { bestA = A; bestD = D; }
                             }
                         }
+                        lim = (bestA == bestD) ? R(1) : bestA / bestD;
                     }
-                    // IEEE division: the winning denominator can be subnormal (a variable
-                    // that is zero up to round-off), where a reciprocal estimate overflows
-                    lim = (bestA == bestD) ? R(1) : bestA / bestD;
                 }
             }
 #pragma unroll
@@ -196,9 +226,10 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
         for (int f = 0; f < 2 * ND; f++) {
             R q = R(2) * face[f][0] * face[f][4] -
                   (face[f][1] * face[f][1] + face[f][2] * face[f][2] + face[f][3] * face[f][3]);
-            bool ok = face[f][0] > R(0) && q > R(0);
+            if (!(face[f][0] > R(0) && q > R(0))) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) face[f][v] = ok ? face[f][v] : c[v];
+                for (int v = 0; v < NV; v++) face[f][v] = c[v];
+            }
         }
     }
 }

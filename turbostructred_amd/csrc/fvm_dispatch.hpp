@@ -8,9 +8,11 @@ namespace tsd {
 
 using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
 
-// dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3
+// dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3, and the
+// fused-dt SSP stages 1 and 2
 struct KernelSet {
     LaunchFn dt = nullptr, s1e = nullptr, s1r = nullptr, s2 = nullptr, s3 = nullptr;
+    LaunchFn f1 = nullptr, f2 = nullptr;
 };
 
 // reconstruction variants: 0 PWC, 1 linear, 2 ENO2, 3 limited linear BJ, 4 limited linear Venkat

@@ -38,12 +38,16 @@ static void fill(int precision, KernelSet* ks) {
         ks->s1r = ks->s1e;
         ks->s2 = launch<RC, LM, MODE_S2, double, double, double>;
         ks->s3 = launch<RC, LM, MODE_S3, double, double, double>;
+        ks->f1 = launch<RC, LM, MODE_F1, double, double, double>;
+        ks->f2 = launch<RC, LM, MODE_F2, double, double, double>;
     } else {
         ks->dt = launch<RC, LM, MODE_DT, float, double, double>;
         ks->s1e = launch<RC, LM, MODE_S1, float, double, double>;
         ks->s1r = launch<RC, LM, MODE_S1, float, double, float>;
         ks->s2 = launch<RC, LM, MODE_S2, float, float, float>;
         ks->s3 = launch<RC, LM, MODE_S3, float, float, double>;
+        ks->f1 = launch<RC, LM, MODE_F1, float, double, float>;
+        ks->f2 = launch<RC, LM, MODE_F2, float, double, float>;
     }
 }
 #endif

@@ -2,24 +2,34 @@
 //
 // One wavefront owns a strip of 64 columns along x (60 output columns plus a halo of two
 // lanes on each side) and marches along y over `ty` output rows, keeping a three-row window of
-// the state in registers.  x-neighbours come from the adjacent lanes with DPP wave shifts, so
-// no LDS is used; every face flux is evaluated once, by the lane on its plus side, and the
-// residual of a cell is accumulated in registers in x-then-y order (ERK:1232,1312) before the
-// stage update is applied and the row is stored with coalesced SoA writes.
+// the state in registers and the next row in flight (prefetched one iteration ahead).
+// x-neighbours come from the adjacent lanes with DPP wave shifts, so no LDS is used; every face
+// flux is evaluated once, by the lane on its plus side, and the residual of a cell is
+// accumulated in registers in x-then-y order (ERK:1232,1312) before the stage update is applied
+// and the row is stored with coalesced SoA writes.
 //
-// Modes: MODE_DT computes the spectral radius of every cell from the face eigenvalues, takes
-// min(V/SR) over the grid (wave shuffle, LDS across waves, one atomic per block) and the last
-// block to finish turns it into dt (ERK:1427-1437).  The stage modes apply forward Euler / SSP
-// stage 1, SSP stage 2 and SSP stage 3 with that dt.
+// Modes
+//   MODE_DT  spectral radius of every cell from the face eigenvalues, min(V/SR) over the grid
+//            (wave shuffle, LDS across waves, one atomic per block); the last block to finish
+//            turns it into dt (ERK:1427-1437).  Used before forward Euler.
+//   MODE_S1  U1 = U + dt L(U)          forward Euler (kernel.h:319-331) / SSP stage 1
+//   MODE_S2  U2 = 3/4 Un + 1/4 (U1 + dt L(U1))
+//   MODE_S3  U  = 1/3 Un + 2/3 (U2 + dt L(U2))   (written in place over Un: Un is read
+//            point-wise by the thread that overwrites it)
+//   MODE_F1  SSP stage 1 with the dt pass fused in: writes L(Un) = R/V and does the dt
+//            reduction of MODE_DT from the same face eigenvalues.
+//   MODE_F2  SSP stage 2 reading Un and L(Un): its stencil values are U1 = Un + dt L(Un)
+//            formed on the fly, so U1 never goes to HBM.
 #pragma once
 
 #include "fvm_device.hpp"
 
 namespace tsd {
 
-enum : int { MODE_DT = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3 };
+enum : int { MODE_DT = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3, MODE_F1 = 4, MODE_F2 = 5 };
 constexpr int STRIP = 60;   // output columns per wavefront
-constexpr int ZPLANES = 8;  // output z-planes per 3-D workgroup (one wavefront each)
+constexpr int ZW = 8;         // wavefronts (z-planes) per 3-D workgroup
+constexpr int ZOUT = ZW - 2;  // output z-planes per 3-D workgroup
 constexpr int MAX_BC = 8;
 
 // Device-resident control block of one solve: the step bookkeeping of StepInfo (kernel.h:56) and
@@ -47,6 +57,7 @@ struct GridArgs {
     const double* rc[3];  // 1/(x_{i+1} - x_{i-1}), interior cells
     const double* rl[3];  // 1/(x_i - x_{i-1})
     const double* rr[3];  // 1/(x_{i+1} - x_i)
+    const double* ih[3];  // 1/h, ghost layer included
     double eps2;          // Venkatakrishnan eps^2 (uniform grids)
     double gamma, eps, cfl;
     BCDev bc[MAX_BC];
@@ -57,6 +68,7 @@ struct GridArgs {
 struct KArgs {
     GridArgs g;
     const void* uin;   // stencil input planes (TIn)
+    const void* lin;   // L(Un) planes (TOut type), MODE_F2
     const double* un;  // U^n planes (SSP stages 2, 3)
     void* uout;        // output planes (TOut)
     size_t plane;      // cells per plane
@@ -122,21 +134,77 @@ __device__ inline int bc_marker(const GridArgs& g, int side, int face_index) {
 }
 
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
 // V/SR of one cell; a NaN or negative spectral radius (non-physical state) becomes 0 so that
 // the step fails with dt = 0 instead of silently skipping the cell in the minimum.
 __device__ inline double dt_candidate(double vol, double sr) {
     double q = vol / sr;
     return (q > 0.0) ? q : 0.0;
 }
-__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Block min of V/SR, one atomic per block, and the last block to finish turns the global
+// minimum into dt: dt = min(V/SR) * CFL (ERK:1430), clamped to the next snapshot (ERK:1433),
+// then advances Time / Iteration and evaluates the stop rule of Kernel::Run (kernel.h:1834-1846).
+__device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block_min, double* red,
+                                          int lane, int wave, int nwaves) {
+    double m = block_min;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nwaves; w++) m = __builtin_fmin(m, red[w]);
+        atomicMin(&ctl->min_bits, dbits(m));
+        __threadfence();
+        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
+            double dtn = mn * cfl;
+            double t0 = ctl->time;
+            if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
+            if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
+                ctl->error = 1;
+                ctl->go = 0;
+                ctl->active = 0;
+            } else {
+                double t1 = t0 + dtn;
+                ctl->dt = dtn;
+                ctl->time = t1;
+                long long it = ctl->iter;
+                ctl->iter = it + 1;
+                int sd = ctl->steps_done + 1;
+                ctl->steps_done = sd;
+                long long li = it - ctl->dt_log_base;
+                if (ctl->dt_log && li >= 0 && li < ctl->dt_log_cap) ctl->dt_log[li] = dtn;
+                ctl->go = 1;
+                bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
+                            (sd >= ctl->max_steps);
+                ctl->active = stop ? 0 : 1;
+            }
+            atomicExch(&ctl->min_bits, ~0ull);
+            atomicExch(&ctl->blocks_done, 0u);
+        }
+    }
+}
+
+template <int MODE> struct ModeTraits {
+    static constexpr bool sr = MODE == MODE_DT || MODE == MODE_F1;  // needs V/SR and dt
+    static constexpr bool flux = MODE != MODE_DT;                    // needs fluxes
+    static constexpr bool use_un = MODE == MODE_S2 || MODE == MODE_S3 || MODE == MODE_F2;
+    static constexpr double ca = (MODE == MODE_S3) ? 1.0 / 3.0 : 0.75;
+    static constexpr double cb = (MODE == MODE_S3) ? 2.0 / 3.0 : 0.25;
+};
 
 // ---- the 1-D / 2-D stage kernel -----------------------------------------------------------------
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
 __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
+    using MT = ModeTraits<MODE>;
     StepCtl* ctl = a.ctl;
     __shared__ double red[4];
-    if constexpr (MODE == MODE_DT) {
+    if constexpr (MT::sr) {
         if (!ctl->active) {
             if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
             return;
@@ -150,8 +218,8 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     const int nwaves = blockDim.x >> 6;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
     const bool tile_ok = tile < a.ntiles;
-    if constexpr (MODE != MODE_DT) {
-        if (!tile_ok) return;  // no block-level synchronisation in the stage modes
+    if constexpr (!MT::sr) {
+        if (!tile_ok) return;  // no block-level synchronisation in these modes
     }
     const int strip = tile_ok ? tile % a.nstrips : 0;
     const int yb = tile_ok ? tile / a.nstrips : 0;
@@ -162,12 +230,15 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     int xs, xside;
     map_index(xg, nx, g.periodic[0], xs, xside);
     const bool out_lane = tile_ok && lane >= 2 && lane < 2 + STRIP && xg < nx;
+    // lane holding the right ghost column: its minus-face flux is the right boundary face
+    const bool ghost_x_lane = tile_ok && xg == nx && lane >= 2 && lane <= 62;
 
     const R gamma = R(g.gamma), eps = R(g.eps);
-    const double dt = (MODE == MODE_DT) ? 0.0 : ctl->dt;
+    const double dt = MT::sr ? 0.0 : ctl->dt;
 
     CellGeo<DIM, R> geo;
     const double hx = g.h[0][xs + 1];
+    const double ihx = g.ih[0][xs + 1];
     geo.hh[0] = R(0.5 * hx);
     geo.rc[0] = R(g.rc[0][xs]);
     geo.rl[0] = R(g.rl[0][xs]);
@@ -175,15 +246,22 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     geo.eps2 = R(g.eps2);
 
     const TIn* __restrict__ uin = (const TIn*)a.uin;
+    const TOut* __restrict__ lin = (const TOut*)a.lin;
+    TOut* uo = (TOut*)a.uout;
     const size_t P = a.plane;
 
-    // load one row of the stencil input (row index may be a ghost row)
+    // one stencil value: the input state at a (possibly ghost) cell of row r
     auto load_row = [&](int r, R (&u)[NV]) {
         int rs = 0, yside = 0;
         if constexpr (DIM > 1) map_index(r, ny, g.periodic[1], rs, yside);
         size_t off = (size_t)rs * nx + xs;
+        if constexpr (MODE == MODE_F2) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+            for (int v = 0; v < NV; v++) u[v] = R(double(uin[v * P + off]) + dt * double(lin[v * P + off]));
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+        }
         if (xside != 0) {
             R t[NV];
             bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, rs)], gamma, 0, u, t);
@@ -197,12 +275,30 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     };
 
+    // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
+    auto finish = [&](size_t off, const R (&u)[NV], const R (&res)[NV], const double (&un)[NV],
+                      double ivol) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            if constexpr (MODE == MODE_F1) {
+                if (out_lane) uo[v * P + off] = TOut(double(res[v]) * ivol);
+            } else {
+                double q = double(u[v]) + dt * (double(res[v]) * ivol);
+                if constexpr (MT::use_un) q = MT::ca * un[v] + MT::cb * q;
+                if (out_lane) uo[v * P + off] = TOut(q);
+            }
+        }
+    };
+
     double block_min = __builtin_inf();
 
     if constexpr (DIM == 1) {
         // ---- one row ----
         R c[NV], nb[2][NV], face[2][NV];
         load_row(0, c);
+        double un[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) un[v] = MT::use_un ? a.un[v * P + xs] : 0.0;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             nb[0][v] = dpp_from_left(c[v]);
@@ -220,8 +316,11 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
             if (xg == nx) bc_image(g.bc[bc_marker(g, 1, 0)], gamma, 0, UL, UR);
         }
         const R Sx = R(1);
-        if constexpr (MODE == MODE_DT) {
-            R lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+        R lam;
+        R F[NV];
+        if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
+        else lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+        if constexpr (MT::sr) {
             R lam_r = dpp_from_right(lam);
             double sr;
             {
@@ -229,41 +328,47 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
                 sr = double(Sx * lam) + double(Sx * lam_r);
             }
             if (out_lane) block_min = dt_candidate(hx, sr);
-            if (tile_ok && xg == nx && lane >= 2 && lane <= 62)
-                block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1], double(Sx * lam)));
-        } else {
-            R F[NV], lam;
-            face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
-            const double ivol = 1.0 / hx;
-            TOut* __restrict__ uo = (TOut*)a.uout;
+            if (ghost_x_lane) block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1], double(Sx * lam)));
+        }
+        if constexpr (MT::flux) {
+            R res[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 R Fr = dpp_from_right(F[v]);
-                R res = -((Fr - F[v]) * Sx);
-                double u = double(c[v]);
-                double q = u + dt * (double(res) * ivol);
-                if constexpr (MODE == MODE_S2) q = 0.75 * a.un[v * P + xs] + 0.25 * q;
-                if constexpr (MODE == MODE_S3) q = (1.0 / 3.0) * a.un[v * P + xs] + (2.0 / 3.0) * q;
-                if (out_lane) uo[v * P + xs] = TOut(q);
+                res[v] = -((Fr - F[v]) * Sx);
             }
+            finish(xs, c, res, un, ihx);
         }
     } else {
         // ---- march along y ----
         const int y0 = yb * a.ty;
         const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
         R Um[NV], U0[NV], Up[NV];
-        R ytop[NV];  // +y face state of the previously reconstructed row
-        R gbot[NV];  // y-flux through the bottom face of the pending row
-        R dfx[NV];   // x part of the residual of the pending row
-        double srp = 0.0;  // spectral radius of the pending row without its top face
+        R Pn[NV];               // prefetched stencil row (consumed as Up one iteration later)
+        double Nc[NV], Nn[NV];  // U^n of the row finalized in this / the next iteration
+        R ytop[NV];             // +y face state of the previously reconstructed row
+        R gbot[NV];             // y-flux through the bottom face of the pending row
+        R dfx[NV];              // x part of the residual of the pending row
+        double srp = 0.0;       // spectral radius of the pending row without its top face
         R lam_gb = R(0);
         load_row(y0 - 2, Um);
         load_row(y0 - 1, U0);
+        load_row(y0, Pn);
+#pragma unroll
+        for (int v = 0; v < NV; v++) Nn[v] = 0.0;
         const int nit = yend - y0 + 2;
-        TOut* __restrict__ uo = (TOut*)a.uout;
         for (int t = 0; t < nit; t++) {
             const int rc = y0 - 1 + t;  // row being reconstructed
-            load_row(rc + 1, Up);
+#pragma unroll
+            for (int v = 0; v < NV; v++) { Up[v] = Pn[v]; Nc[v] = Nn[v]; }
+            if (t + 1 < nit) load_row(rc + 2, Pn);
+            if constexpr (MT::use_un) {
+                if (rc >= y0 && rc < yend) {
+                    const size_t offn = (size_t)rc * nx + xs;
+#pragma unroll
+                    for (int v = 0; v < NV; v++) Nn[v] = a.un[v * P + offn];
+                }
+            }
             int rcs, rcside;
             map_index(rc, ny, g.periodic[1], rcs, rcside);
             const double hy = g.h[1][rcs + 1];
@@ -290,8 +395,10 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
                     if (rc == ny) bc_image(g.bc[bc_marker(g, 3, xs)], gamma, 1, GL, GR);
                 }
                 const R Sy = R(hx);  // face area of a y-face (ERK:1268)
-                if constexpr (MODE == MODE_DT) {
-                    R lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+                R G[NV], lam;
+                if constexpr (MT::flux) face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
+                else lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+                if constexpr (MT::sr) {
                     if (t >= 2) {
                         double sr;
                         {
@@ -303,22 +410,16 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
                     if (rc == ny && out_lane)  // top ghost row: only this face (ERK:1305)
                         block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][ny + 1], double(Sy * lam)));
                     lam_gb = lam;
-                } else {
-                    R G[NV], lam;
-                    face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
+                }
+                if constexpr (MT::flux) {
                     if (t >= 2) {
                         // finalize row rc-1 = the row held in Um
                         const int rf = rc - 1;
-                        const double ivol = 1.0 / (hx * g.h[1][rf + 1]);
-                        const size_t off = (size_t)rf * nx + xs;
+                        const double ivol = ihx * g.ih[1][rf + 1];
+                        R res[NV];
 #pragma unroll
-                        for (int v = 0; v < NV; v++) {
-                            R res = dfx[v] - (G[v] - gbot[v]) * Sy;
-                            double q = double(Um[v]) + dt * (double(res) * ivol);
-                            if constexpr (MODE == MODE_S2) q = 0.75 * a.un[v * P + off] + 0.25 * q;
-                            if constexpr (MODE == MODE_S3) q = (1.0 / 3.0) * a.un[v * P + off] + (2.0 / 3.0) * q;
-                            if (out_lane) uo[v * P + off] = TOut(q);
-                        }
+                        for (int v = 0; v < NV; v++) res[v] = dfx[v] - (G[v] - gbot[v]) * Sy;
+                        finish((size_t)rf * nx + xs, Um, res, Nc, ivol);
                     }
 #pragma unroll
                     for (int v = 0; v < NV; v++) gbot[v] = G[v];
@@ -336,19 +437,20 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
                         if (xg == nx) bc_image(g.bc[bc_marker(g, 1, rcs)], gamma, 0, UL, UR);
                     }
                     const R Sx = R(hy);  // face area of an x-face (ERK:1189)
-                    if constexpr (MODE == MODE_DT) {
-                        R lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
-                        R lam_r = dpp_from_right(lam);
+                    R F[NV], lamx;
+                    if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lamx);
+                    else lamx = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+                    if constexpr (MT::sr) {
+                        R lam_r = dpp_from_right(lamx);
                         {
 #pragma clang fp contract(off)
-                            srp = double(Sx * lam) + double(Sx * lam_r);
+                            srp = double(Sx * lamx) + double(Sx * lam_r);
                             srp = srp + double(Sy * lam_gb);
                         }
-                        if (tile_ok && xg == nx && lane >= 2 && lane <= 62)  // right ghost column (ERK:1225)
-                            block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy, double(Sx * lam)));
-                    } else {
-                        R F[NV], lam;
-                        face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
+                        if (ghost_x_lane)  // right ghost column (ERK:1225)
+                            block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy, double(Sx * lamx)));
+                    }
+                    if constexpr (MT::flux) {
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
                             R Fr = dpp_from_right(F[v]);
@@ -366,48 +468,7 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     }
 
-    if constexpr (MODE == MODE_DT) {
-        // wave min, then block min, then one atomic per block
-        double m = block_min;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
-        if (lane == 0) red[wave] = m;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < nwaves; w++) m = __builtin_fmin(m, red[w]);
-            atomicMin(&ctl->min_bits, dbits(m));
-            __threadfence();
-            unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
-            if (prev == gridDim.x - 1) {
-                __threadfence();
-                double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
-                double dtn = mn * g.cfl;  // ERK:1430
-                double t0 = ctl->time;
-                if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;  // ERK:1433
-                if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
-                    ctl->error = 1;
-                    ctl->go = 0;
-                    ctl->active = 0;
-                } else {
-                    double t1 = t0 + dtn;
-                    ctl->dt = dtn;
-                    ctl->time = t1;
-                    long long it = ctl->iter;
-                    ctl->iter = it + 1;
-                    int sd = ctl->steps_done + 1;
-                    ctl->steps_done = sd;
-                    long long li = it - ctl->dt_log_base;
-                    if (ctl->dt_log && li >= 0 && li < ctl->dt_log_cap) ctl->dt_log[li] = dtn;
-                    ctl->go = 1;
-                    bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
-                                (sd >= ctl->max_steps);
-                    ctl->active = stop ? 0 : 1;
-                }
-                atomicExch(&ctl->min_bits, ~0ull);
-                atomicExch(&ctl->blocks_done, 0u);
-            }
-        }
-    }
+    if constexpr (MT::sr) dt_reduce_finalize(ctl, g.cfl, block_min, red, lane, wave, nwaves);
 }
 
 }  // namespace tsd

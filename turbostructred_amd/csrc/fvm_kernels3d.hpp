@@ -1,0 +1,303 @@
+// fvm_kernels3d.hpp -- fused finite-volume stage kernel for 3-D grids (gfx950).
+//
+// A workgroup is ZW = 8 wavefronts, one per z-plane k = k0-1+w; wavefronts 1..6 produce output
+// (ZOUT = 6 planes per workgroup) and the two edge wavefronts only supply reconstructions (each
+// loads the one extra centre row its outer z-neighbour needs).  Inside a plane the scheme is the
+// 2-D one: 64 lanes along x (60 output columns) and a register window marching along y.
+// z-neighbours travel through LDS once per row: the centre row of the state, then the z+ face
+// state of each plane, then the z-face flux -- three barriers per row.  Each z-face flux is
+// evaluated once, by the wavefront above it.  The residual of a cell is accumulated x, then y,
+// then z (ERK:1232,1312,1392) and its spectral radius x-, x+, y-, y+, z-, z+ (ERK:1225-1248,
+// 1305-1328, 1385-1408).
+#pragma once
+
+#include "fvm_kernels.hpp"
+
+namespace tsd {
+
+template <int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+__global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
+    using MT = ModeTraits<MODE>;
+    StepCtl* ctl = a.ctl;
+    __shared__ R lds_u[ZW][NV][64];  // centre row of every plane
+    __shared__ R lds_z[ZW][NV][64];  // z+ face state of every plane
+    __shared__ R lds_h[ZW][NV][64];  // flux through the z-face below every plane
+    __shared__ R lds_l[ZW][64];      // eigenvalue of that face
+    __shared__ double red[ZW];
+    if constexpr (MT::sr) {
+        if (!ctl->active) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
+            return;
+        }
+    } else {
+        if (!ctl->go) return;
+    }
+    const GridArgs& g = a.g;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nyb = (ny + a.ty - 1) / a.ty;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);  // one tile per workgroup
+    const int strip = tile % a.nstrips;
+    const int yb = (tile / a.nstrips) % nyb;
+    const int zb = tile / (a.nstrips * nyb);
+    const int k = zb * ZOUT - 1 + w;  // this wavefront's plane (may be a ghost plane)
+    int ks, kside;
+    map_index(k, nz, g.periodic[2], ks, kside);
+    const bool out_plane = w >= 1 && w <= ZOUT && k < nz;
+
+    const int xg = strip * STRIP - 2 + lane;
+    int xs, xside;
+    map_index(xg, nx, g.periodic[0], xs, xside);
+    const bool x_out = lane >= 2 && lane < 2 + STRIP && xg < nx;
+    const bool out_lane = out_plane && x_out;
+    const bool ghost_x_lane = out_plane && xg == nx && lane >= 2 && lane <= 62;
+
+    const R gamma = R(g.gamma), eps = R(g.eps);
+    const double dt = MT::sr ? 0.0 : ctl->dt;
+
+    CellGeo<3, R> geo;
+    const double hx = g.h[0][xs + 1], ihx = g.ih[0][xs + 1];
+    const double hz = g.h[2][ks + 1], ihz = g.ih[2][ks + 1];
+    geo.hh[0] = R(0.5 * hx);
+    geo.rc[0] = R(g.rc[0][xs]);
+    geo.rl[0] = R(g.rl[0][xs]);
+    geo.rr[0] = R(g.rr[0][xs]);
+    geo.hh[2] = R(0.5 * hz);
+    geo.rc[2] = R(g.rc[2][ks]);
+    geo.rl[2] = R(g.rl[2][ks]);
+    geo.rr[2] = R(g.rr[2][ks]);
+    geo.eps2 = R(g.eps2);
+
+    const TIn* __restrict__ uin = (const TIn*)a.uin;
+    const TOut* __restrict__ lin = (const TOut*)a.lin;
+    TOut* uo = (TOut*)a.uout;
+    const size_t P = a.plane;
+
+    // stencil input at (xg, row r, plane kk); ghosts take their boundary image
+    auto load_cell = [&](int r, int kk, R (&u)[NV]) {
+        int rs, yside, kks, zside;
+        map_index(r, ny, g.periodic[1], rs, yside);
+        map_index(kk, nz, g.periodic[2], kks, zside);
+        size_t off = ((size_t)kks * ny + rs) * nx + xs;
+        if constexpr (MODE == MODE_F2) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = R(double(uin[v * P + off]) + dt * double(lin[v * P + off]));
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+        }
+        int axis = -1, side = 0, fidx = 0;
+        if (xside != 0) { axis = 0; side = xside < 0 ? 0 : 1; fidx = kks * ny + rs; }
+        else if (yside != 0) { axis = 1; side = yside < 0 ? 2 : 3; fidx = kks * nx + xs; }
+        else if (zside != 0) { axis = 2; side = zside < 0 ? 4 : 5; fidx = rs * nx + xs; }
+        if (axis >= 0) {
+            R t[NV];
+            bc_image(g.bc[bc_marker(g, side, fidx)], gamma, axis, u, t);
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = t[v];
+        }
+    };
+
+    double block_min = __builtin_inf();
+    const int y0 = yb * a.ty;
+    const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
+    R Um[NV], U0[NV], Up[NV], Pn[NV];
+    R Zx[NV];               // centre row of the z-neighbour no wavefront holds (edge wavefronts)
+    double Nc[NV], Nn[NV];  // U^n of the row finalized in this / the next iteration
+    R ytop[NV];             // +y face state of the previous row
+    R gbot[NV];             // y-flux through the bottom face of the pending row
+    R dfx[NV], dhz[NV];     // x and z parts of the residual of the pending row
+    double srp = 0.0;       // x terms + bottom y term of the pending row's spectral radius
+    R lzb = R(0), lzt = R(0);  // z-face eigenvalues of the pending row
+    R lam_gb = R(0);
+    load_cell(y0 - 2, k, Um);
+    load_cell(y0 - 1, k, U0);
+    load_cell(y0, k, Pn);
+#pragma unroll
+    for (int v = 0; v < NV; v++) { Nn[v] = 0.0; dfx[v] = dhz[v] = gbot[v] = R(0); }
+    const bool edge = (w == 0) || (w == ZW - 1);
+    const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
+    const int nit = yend - y0 + 2;
+    for (int t = 0; t < nit; t++) {
+        const int rc = y0 - 1 + t;  // row being reconstructed
+        const bool row_out = rc >= y0 && rc < yend;
+#pragma unroll
+        for (int v = 0; v < NV; v++) { Up[v] = Pn[v]; Nc[v] = Nn[v]; }
+        if (t + 1 < nit) load_cell(rc + 2, k, Pn);
+        if (edge) load_cell(rc, k + zoff, Zx);
+        if constexpr (MT::use_un) {
+            if (out_plane && row_out) {
+                const size_t offn = ((size_t)ks * ny + rc) * nx + xs;
+#pragma unroll
+                for (int v = 0; v < NV; v++) Nn[v] = a.un[v * P + offn];
+            }
+        }
+        int rcs, rcside;
+        map_index(rc, ny, g.periodic[1], rcs, rcside);
+        const double hy = g.h[1][rcs + 1];
+        geo.hh[1] = R(0.5 * hy);
+        geo.rc[1] = R(g.rc[1][rcs]);
+        geo.rl[1] = R(g.rl[1][rcs]);
+        geo.rr[1] = R(g.rr[1][rcs]);
+
+        // (1) share the centre row, reconstruct
+#pragma unroll
+        for (int v = 0; v < NV; v++) lds_u[w][v][lane] = U0[v];
+        __syncthreads();
+        R face[6][NV];
+        {
+            R nb[6][NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                nb[0][v] = dpp_from_left(U0[v]);
+                nb[1][v] = dpp_from_right(U0[v]);
+                nb[2][v] = Um[v];
+                nb[3][v] = Up[v];
+                nb[4][v] = (w == 0) ? Zx[v] : lds_u[w - 1][v][lane];
+                nb[5][v] = (w == ZW - 1) ? Zx[v] : lds_u[w + 1][v][lane];
+            }
+            reconstruct<3, RECON, LIM, R, FLUX != FX_ROE>(U0, nb, geo, face);
+        }
+        // (2) share the z+ face state; the wavefront above each z-face evaluates it
+#pragma unroll
+        for (int v = 0; v < NV; v++) lds_z[w][v][lane] = face[5][v];
+        __syncthreads();
+        const R Sz = R(hx * hy);  // face area of a z-face (ERK:1349)
+        R H[NV], lamz = R(0);
+#pragma unroll
+        for (int v = 0; v < NV; v++) H[v] = R(0);
+        if (w >= 1 && row_out) {
+            R ZL[NV], ZR[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) { ZL[v] = lds_z[w - 1][v][lane]; ZR[v] = face[4][v]; }
+            if (!g.periodic[2]) {
+                if (k == 0) bc_image(g.bc[bc_marker(g, 4, rcs * nx + xs)], gamma, 2, ZR, ZL);
+                if (k == nz) bc_image(g.bc[bc_marker(g, 5, rcs * nx + xs)], gamma, 2, ZL, ZR);
+            }
+            if constexpr (MT::flux) face_flux<FLUX, 2, R>(gamma, eps, ZL, ZR, H, lamz);
+            else lamz = face_eigen<FLUX, 2, R>(gamma, ZL, ZR);
+            if constexpr (MT::flux) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) lds_h[w][v][lane] = H[v];
+            }
+            lds_l[w][lane] = lamz;
+        }
+        // (3) the flux through the face above comes from the wavefront above
+        __syncthreads();
+        R dhz_new[NV], lzb_new = lamz, lzt_new = R(0);
+#pragma unroll
+        for (int v = 0; v < NV; v++) dhz_new[v] = R(0);
+        if (out_plane && row_out) {
+            lzt_new = lds_l[w + 1][lane];
+            if constexpr (MT::flux) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) dhz_new[v] = (lds_h[w + 1][v][lane] - H[v]) * Sz;
+            }
+        }
+        if constexpr (MT::sr) {
+            // top ghost plane: only the face below it (ERK:1385)
+            if (row_out && k == nz && w >= 1 && x_out)
+                block_min = __builtin_fmin(block_min, dt_candidate(hx * hy * g.h[2][nz + 1], double(Sz * lamz)));
+        }
+
+        if (out_plane && t >= 1) {
+            // y-face between rows rc-1 and rc
+            R GL[NV], GR[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) { GL[v] = ytop[v]; GR[v] = face[2][v]; }
+            if (!g.periodic[1]) {
+                if (rc == 0) bc_image(g.bc[bc_marker(g, 2, ks * nx + xs)], gamma, 1, GR, GL);
+                if (rc == ny) bc_image(g.bc[bc_marker(g, 3, ks * nx + xs)], gamma, 1, GL, GR);
+            }
+            const R Sy = R(hx * hz);  // face area of a y-face (ERK:1269)
+            R G[NV], lam;
+            if constexpr (MT::flux) face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
+            else lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+            if constexpr (MT::sr) {
+                if (t >= 2) {
+                    double sr;
+                    {
+#pragma clang fp contract(off)
+                        sr = srp + double(Sy * lam);
+                        sr = sr + double(Sz * lzb);
+                        sr = sr + double(Sz * lzt);
+                    }
+                    if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc] * hz, sr));
+                }
+                if (rc == ny && out_lane)  // top ghost row (ERK:1305)
+                    block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][ny + 1] * hz, double(Sy * lam)));
+                lam_gb = lam;
+            }
+            if constexpr (MT::flux) {
+                if (t >= 2) {
+                    // finalize row rc-1 (held in Um)
+                    const int rf = rc - 1;
+                    const double ivol = ihx * g.ih[1][rf + 1] * ihz;
+                    const size_t off = ((size_t)ks * ny + rf) * nx + xs;
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        R res = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
+                        if constexpr (MODE == MODE_F1) {
+                            if (out_lane) uo[v * P + off] = TOut(double(res) * ivol);
+                        } else {
+                            double q = double(Um[v]) + dt * (double(res) * ivol);
+                            if constexpr (MT::use_un) q = MT::ca * Nc[v] + MT::cb * q;
+                            if (out_lane) uo[v * P + off] = TOut(q);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < NV; v++) gbot[v] = G[v];
+            }
+            if (row_out) {
+                // x-faces of row rc
+                R UL[NV], UR[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    UL[v] = dpp_from_left(face[1][v]);
+                    UR[v] = face[0][v];
+                }
+                if (!g.periodic[0]) {
+                    if (xg == 0) bc_image(g.bc[bc_marker(g, 0, ks * ny + rcs)], gamma, 0, UR, UL);
+                    if (xg == nx) bc_image(g.bc[bc_marker(g, 1, ks * ny + rcs)], gamma, 0, UL, UR);
+                }
+                const R Sx = R(hy * hz);  // face area of an x-face (ERK:1190)
+                R F[NV], lamx;
+                if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lamx);
+                else lamx = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+                if constexpr (MT::sr) {
+                    R lam_r = dpp_from_right(lamx);
+                    {
+#pragma clang fp contract(off)
+                        srp = double(Sx * lamx) + double(Sx * lam_r);
+                        srp = srp + double(Sy * lam_gb);
+                    }
+                    if (ghost_x_lane)  // right ghost column (ERK:1225)
+                        block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy * hz, double(Sx * lamx)));
+                }
+                if constexpr (MT::flux) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        R Fr = dpp_from_right(F[v]);
+                        dfx[v] = -((Fr - F[v]) * Sx);
+                    }
+                }
+            }
+        }
+        // the z part of the row just reconstructed becomes the pending row's
+#pragma unroll
+        for (int v = 0; v < NV; v++) dhz[v] = dhz_new[v];
+        lzb = lzb_new;
+        lzt = lzt_new;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ytop[v] = face[3][v];
+            Um[v] = U0[v];
+            U0[v] = Up[v];
+        }
+    }
+    if constexpr (MT::sr) dt_reduce_finalize(ctl, g.cfl, block_min, red, lane, w, ZW);
+}
+
+}  // namespace tsd

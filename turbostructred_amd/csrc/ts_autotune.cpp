@@ -40,7 +40,7 @@ int tuned_rows_per_tile(const ts_config& c) {
     }
     // default: enough tiles to fill 256 CUs several times over, at most 32 rows
     long long strips = (c.nx + 59) / 60;
-    long long rows = c.ndims == 2 ? c.ny : (long long)c.ny * ((c.nz + 7) / 8);
+    long long rows = c.ndims == 2 ? c.ny : (long long)c.ny * ((c.nz + 5) / 6);
     int ty = 32;
     while (ty > 4 && strips * ((rows + ty - 1) / ty) < 4096) ty /= 2;
     return ty;

@@ -23,6 +23,9 @@ struct ts_ctx {
     void* dU[3] = {nullptr, nullptr, nullptr};
     size_t stage_elem = 8;
     int cur = 0;
+    // SSP-RK3 with the dt pass fused into stage 1 (dU[1] then holds L(U^n)); off when the
+    // environment sets TS_RK3_SEPARATE_DT=1
+    bool fused_dt = true;
 
     // geometry on the device
     std::vector<double*> dbufs;  // everything to free

@@ -303,6 +303,11 @@ int ts_algorithmic_bytes_per_cell(const ts_ctx* c, int kind, double* bytes) {
     if (c->cfg.rk_stages == 1) {
         b[1] = 2 * d;  // read U^n, write U^{n+1}
         b[2] = b[3] = 0;
+    } else if (c->fused_dt) {
+        b[0] = 0;            // no separate dt pass
+        b[1] = d + s;        // read U^n, write L(U^n)
+        b[2] = d + s + s;    // read U^n and L(U^n), write U^2
+        b[3] = s + d + d;    // read U^2, U^n, write U^{n+1}
     } else {
         b[1] = d + s;        // read U^n, write U^1
         b[2] = s + d + s;    // read U^1, U^n, write U^2

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "ts_internal.hpp"
@@ -68,7 +69,7 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     g.cfl = k.cfl;
     bool uniform = true;
     for (int a = 0; a < 3; a++) {
-        std::vector<double> rc, rl, rr;
+        std::vector<double> rc, rl, rr, ih;
         if (a < c->nd) {
             tsr::build_axis(*cfg, a, c->X[a], c->H[a]);
             uniform = uniform && tsr::uniform_axis(c->H[a]);
@@ -89,8 +90,10 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
             rl.assign(1, 0.0);
             rr.assign(1, 0.0);
         }
+        for (double hv : c->H[a]) ih.push_back(1.0 / hv);
         int r;
         if ((r = upload(c, c->H[a].data(), c->H[a].size(), &g.h[a]))) return r;
+        if ((r = upload(c, ih.data(), ih.size(), &g.ih[a]))) return r;
         if ((r = upload(c, rc.data(), rc.size(), &g.rc[a]))) return r;
         if ((r = upload(c, rl.data(), rl.size(), &g.rl[a]))) return r;
         if ((r = upload(c, rr.data(), rr.size(), &g.rr[a]))) return r;
@@ -169,12 +172,14 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->ty = tsr::tuned_rows_per_tile(*cfg);
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
         int nyb = (c->n[1] + c->ty - 1) / c->ty;
-        int nzb = (c->n[2] + ZPLANES - 1) / ZPLANES;
+        int nzb = (c->n[2] + ZOUT - 1) / ZOUT;
         c->ntiles = c->nstrips * nyb * nzb;
-        c->block_dim = dim3(64 * ZPLANES);
+        c->block_dim = dim3(64 * ZW);
         c->grid_dim = dim3(c->ntiles);
     }
     c->host_stage.resize(c->ncell * 5);
+    const char* sep = getenv("TS_RK3_SEPARATE_DT");
+    c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2;
     return TS_OK;
 }
 
@@ -268,6 +273,17 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[4], int& n
         L[n++] = {c->ks.dt, a, 0};
         a.uout = c->dU[out];
         L[n++] = {c->ks.s1e, a, 1};
+    } else if (c->fused_dt) {
+        a.uin = c->dU[0];
+        a.uout = c->dU[1];
+        L[n++] = {c->ks.f1, a, 1};  // L(U^n) and dt
+        a.lin = c->dU[1];
+        a.uout = c->dU[2];
+        L[n++] = {c->ks.f2, a, 2};  // U2 from U^n + dt L(U^n) on the fly
+        a.lin = nullptr;
+        a.uin = c->dU[2];
+        a.uout = c->dU[0];
+        L[n++] = {c->ks.s3, a, 3};
     } else {
         a.uin = c->dU[0];
         L[n++] = {c->ks.dt, a, 0};

@@ -1,0 +1,151 @@
+"""Batched solves (persistent work-queue kernels over many problems): parity with single-solve
+runs and with the oracle, and the same result alone and in a mix."""
+import numpy as np
+import pytest
+
+from turbostructred_amd import capi
+from oracle import pyoracle
+import util
+
+pytestmark = pytest.mark.gpu
+
+
+def tube(n, seed, gamma=1.4, flux=capi.RIEMANN_GODUNOV, precision=capi.FP64, recon=capi.RECON_LINEAR_LIM):
+    r = np.random.default_rng(seed)
+    cfg = capi.make_config(ndims=1, nx=n, cfl=0.45, gamma=gamma, riemann=flux, recon=recon, rk_stages=3,
+                           precision=precision)
+    left = (r.uniform(0.1, 2), r.uniform(-1, 1), r.uniform(0.1, 10))
+    right = (r.uniform(0.1, 2), r.uniform(-1, 1), r.uniform(0.1, 10))
+    return cfg, util.sod(cfg, x0=r.uniform(0.3, 0.7), left=left, right=right)
+
+
+def quad(n, m, seed, precision=capi.FP64):
+    r = np.random.default_rng(seed)
+    cfg = capi.make_config(ndims=2, nx=n, ny=m, cfl=0.45, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=3, precision=precision)
+    st = {q: (r.uniform(0.1, 2), r.uniform(-1.5, 1.5) / 2, r.uniform(-1.5, 1.5) / 2, r.uniform(0.05, 2))
+          for q in ("NE", "NW", "SW", "SE")}
+    return cfg, util.quadrants(cfg, st)
+
+
+def run_alone(cfg, U, steps, t_end=0.0):
+    s = capi.Solver(cfg)
+    s.set_state(U)
+    if t_end > 0:
+        info = s.step(steps, t_end, t_end)
+    else:
+        info = s.step(steps)
+    return s.get_state(), info
+
+
+def drain(b, tickets, chunk=4, max_runs=2000):
+    out = {}
+    for _ in range(max_runs):
+        b.run(chunk)
+        for t in list(tickets):
+            done, st, info = b.poll(t)
+            if done:
+                out[t] = (st, info)
+                tickets.remove(t)
+        if not tickets:
+            return out
+    raise AssertionError(f"batch did not drain: {len(tickets)} solves still pending")
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32_RESIDUAL])
+def test_batched_equals_single_solves_1d(precision):
+    probs = [tube(n, s, gamma=g, precision=precision) for s, (n, g) in
+             enumerate([(64, 1.4), (97, 1.67), (200, 1.4), (61, 1.4), (500, 1.67), (1000, 1.4)])]
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    tickets = [b.submit(cfg, U, 40) for cfg, U in probs]
+    res = drain(b, list(tickets))
+    for (cfg, U), t in zip(probs, tickets):
+        st, info = res[t]
+        alone, ainfo = run_alone(cfg, U, 40)
+        assert info.steps_done == 40 and ainfo.steps_done == 40
+        np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, alone))
+        assert info.time == ainfo.time
+
+
+def test_batched_2d_fp64_matches_oracle():
+    probs = [quad(n, m, s) for s, (n, m) in enumerate([(64, 64), (90, 70), (130, 61)])]
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    tickets = [b.submit(cfg, U, 15) for cfg, U in probs]
+    res = drain(b, list(tickets))
+    for (cfg, U), t in zip(probs, tickets):
+        st, info = res[t]
+        o = pyoracle.Oracle(cfg)
+        o.set_state(U)
+        oi = o.step(15)
+        err = util.normwise_rel(capi.interior(cfg, st), capi.interior(cfg, o.get_state()))
+        assert np.all(err <= 1e-12), err
+        assert abs(info.time - oi.time) <= 1e-12 * oi.time
+
+
+def test_alone_equals_in_the_mix_bitwise():
+    """continuous batching: solves join while others run; each result is bitwise the one it
+    gets alone"""
+    probs = [quad(n, m, 10 + s, precision=capi.FP32_RESIDUAL) for s, (n, m) in
+             enumerate([(64, 64), (100, 80), (150, 150), (70, 200)])]
+    steps = [30, 12, 25, 18]
+    alone = []
+    for (cfg, U), k in zip(probs, steps):
+        b = capi.Batch(hbm_budget_bytes=1 << 28)
+        t = b.submit(cfg, U, k)
+        alone.append(drain(b, [t])[t])
+        b.close()
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    t0 = b.submit(*probs[0], steps[0])
+    b.run(5)  # the first solve is mid-flight when the others arrive
+    ts = [t0] + [b.submit(cfg, U, k) for (cfg, U), k in zip(probs[1:], steps[1:])]
+    mixed = drain(b, list(ts), chunk=3)
+    for t, (a_st, a_info), (cfg, _) in zip(ts, alone, probs):
+        m_st, m_info = mixed[t]
+        np.testing.assert_array_equal(capi.interior(cfg, m_st), capi.interior(cfg, a_st))
+        assert m_info.steps_done == a_info.steps_done and m_info.time == a_info.time
+
+
+def test_t_end_clamp_and_admission_by_budget():
+    # each solve stops exactly at its own t_end; a budget that holds only a few solves at a time
+    # still completes all of them (admission waits for memory)
+    probs = [tube(400, 30 + s) for s in range(10)]
+    t_ends = [0.01 * (1 + s % 4) for s in range(10)]
+    one = pyoracle.Oracle(probs[0][0])
+    b = capi.Batch(hbm_budget_bytes=3 * 400 * 5 * 8 * 3 + (1 << 16))
+    tickets = [b.submit(cfg, U, 100000, te) for (cfg, U), te in zip(probs, t_ends)]
+    res = drain(b, list(tickets), chunk=16)
+    for t, te in zip(tickets, t_ends):
+        assert res[t][1].time == te
+    del one
+
+
+def test_failed_solve_does_not_disturb_the_others():
+    good = [tube(300, 50 + s) for s in range(3)]
+    bad_cfg, bad_U = tube(300, 99)
+    bad_U = bad_U.reshape(-1, 5).copy()
+    bad_U[100, 0] = np.nan
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    tickets = [b.submit(cfg, U, 30) for cfg, U in good]
+    tb = b.submit(bad_cfg, bad_U, 30)
+    ref = [run_alone(cfg, U, 30)[0] for cfg, U in good]
+    done_bad = False
+    results = {}
+    for _ in range(500):
+        b.run(4)
+        for t in list(tickets):
+            d, st, info = b.poll(t)
+            if d:
+                results[t] = st
+                tickets.remove(t)
+        if not done_bad:
+            try:
+                d, _, _ = b.poll(tb)
+                done_bad = d
+            except capi.TSError as e:
+                assert e.code == capi.TS_ERR_UNPHYSICAL
+                done_bad = True
+        if not tickets and done_bad:
+            break
+    assert done_bad and not tickets
+    for t, (cfg, _), r in zip(sorted(results), good, ref):
+        np.testing.assert_array_equal(capi.interior(cfg, results[t]), capi.interior(cfg, r))

@@ -74,8 +74,13 @@ def quadrants(cfg, states=LAX_LIU_3, xc=0.5, yc=0.5):
 
 
 def normwise_rel(a, b):
-    """max |a-b| / max |b| per conserved variable over the interior."""
+    """max |a-b| / scale per conserved variable over the interior.  The scale of rho and rho*E
+    is their own max |b|; the three momentum components share the scale of the momentum vector
+    (max over the three), so a component that is zero by symmetry is measured against the
+    momentum magnitude rather than against its own round-off."""
     a = np.asarray(a).reshape(-1, 5)
     b = np.asarray(b).reshape(-1, 5)
-    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    scale = np.abs(b).max(axis=0)
+    scale[1:4] = scale[1:4].max()
+    scale = np.maximum(scale, 1e-300)
     return (np.abs(a - b).max(axis=0) / scale)

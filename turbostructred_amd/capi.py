@@ -302,3 +302,70 @@ class Solver:
         b = C.c_size_t()
         self._check(self.lib.ts_memory_footprint(self.h, C.byref(b)))
         return b.value
+
+
+class Batch:
+    """Owner of one ts_batch: many independent solves advanced together (continuous batching)."""
+
+    def __init__(self, hbm_budget_bytes=8 << 30, max_active=1 << 20, use_graphs=True, device=0):
+        self.lib = load()
+        bc = BatchConfig()
+        bc.device = device
+        bc.hbm_budget_bytes = hbm_budget_bytes
+        bc.max_active = max_active
+        bc.use_graphs = int(use_graphs)
+        bc.autotune = 0
+        h = C.c_void_p()
+        rc = self.lib.ts_batch_create(C.byref(bc), C.byref(h))
+        if rc != TS_OK:
+            raise TSError(rc, "ts_batch_create failed")
+        self.h = h
+        self._cfgs = {}
+
+    def _check(self, rc):
+        if rc != TS_OK:
+            msg = self.lib.ts_batch_last_error(self.h)
+            raise TSError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self.h:
+            self.lib.ts_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cfg, aos, max_steps, t_end=0.0):
+        a = np.ascontiguousarray(aos, dtype=np.float64).ravel()
+        t = C.c_longlong()
+        self._check(self.lib.ts_batch_submit(self.h, C.byref(cfg), _dp(a), max_steps, t_end, C.byref(t)))
+        self._cfgs[t.value] = cfg
+        return t.value
+
+    def run(self, n_steps):
+        pending = C.c_int()
+        self._check(self.lib.ts_batch_run(self.h, n_steps, C.byref(pending)))
+        return pending.value
+
+    def poll(self, ticket):
+        """(done, state or None, StepInfo or None); raises TSError for a failed solve"""
+        cfg = self._cfgs[ticket]
+        out = np.empty(int(np.prod(state_shape(cfg))), dtype=np.float64)
+        done = C.c_int()
+        info = StepInfo()
+        rc = self.lib.ts_batch_poll(self.h, ticket, C.byref(done), _dp(out), C.byref(info))
+        if done.value:
+            del self._cfgs[ticket]
+        self._check(rc)
+        if not done.value:
+            return False, None, None
+        return True, out, info
+
+    def start_server(self):
+        self._check(self.lib.ts_batch_start_server(self.h))
+
+    def stop_server(self):
+        self._check(self.lib.ts_batch_stop_server(self.h))

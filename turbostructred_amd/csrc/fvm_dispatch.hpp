@@ -7,12 +7,18 @@
 namespace tsd {
 
 using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
+// batched (persistent, work-queue) launch: descriptors, tile table, queue, counter slot, slots,
+// number of 256-thread blocks
+using BatchLaunchFn = void (*)(const KArgs*, const int2*, BatchQueue*, int, int, int, hipStream_t);
+using OccupancyFn = int (*)();  // resident 256-thread blocks per CU of the batched kernels
 
 // dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3, and the
 // fused-dt SSP stages 1 and 2
 struct KernelSet {
     LaunchFn dt = nullptr, s1e = nullptr, s1r = nullptr, s2 = nullptr, s3 = nullptr;
     LaunchFn f1 = nullptr, f2 = nullptr;
+    BatchLaunchFn bf1 = nullptr, bf2 = nullptr, bs3 = nullptr;  // fused-dt SSP-RK3, batched
+    OccupancyFn bocc = nullptr;
 };
 
 // reconstruction variants: 0 PWC, 1 linear, 2 ENO2, 3 limited linear BJ, 4 limited linear Venkat

@@ -30,6 +30,25 @@ static void launch(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
 #endif
 }
 
+#if TS_DIM != 3
+template <int RC, int LM, int MODE, typename R, typename TIn, typename TOut>
+static void launch_batched(const KArgs* d, const int2* t, BatchQueue* q, int slot, int nslots, int nblk,
+                           hipStream_t s) {
+    hipLaunchKernelGGL((fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE, R, TIn, TOut>), dim3(nblk), dim3(256), 0, s,
+                       d, t, q, slot, nslots);
+}
+template <int RC, int LM, typename R, typename S>
+static int batched_occupancy() {
+    int a = 0, b = 0, c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE_F1, R, double, S>, 256, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE_F2, R, double, S>, 256, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE_S3, R, S, double>, 256, 0) != hipSuccess) return 1;
+    int m = a < b ? a : b;
+    m = m < c ? m : c;
+    return m > 0 ? m : 1;
+}
+#endif
+
 template <int RC, int LM>
 static void fill(int precision, KernelSet* ks) {
     if (precision == 0) {
@@ -40,6 +59,12 @@ static void fill(int precision, KernelSet* ks) {
         ks->s3 = launch<RC, LM, MODE_S3, double, double, double>;
         ks->f1 = launch<RC, LM, MODE_F1, double, double, double>;
         ks->f2 = launch<RC, LM, MODE_F2, double, double, double>;
+#if TS_DIM != 3
+        ks->bf1 = launch_batched<RC, LM, MODE_F1, double, double, double>;
+        ks->bf2 = launch_batched<RC, LM, MODE_F2, double, double, double>;
+        ks->bs3 = launch_batched<RC, LM, MODE_S3, double, double, double>;
+        ks->bocc = batched_occupancy<RC, LM, double, double>;
+#endif
     } else {
         ks->dt = launch<RC, LM, MODE_DT, float, double, double>;
         ks->s1e = launch<RC, LM, MODE_S1, float, double, double>;
@@ -48,6 +73,12 @@ static void fill(int precision, KernelSet* ks) {
         ks->s3 = launch<RC, LM, MODE_S3, float, float, double>;
         ks->f1 = launch<RC, LM, MODE_F1, float, double, float>;
         ks->f2 = launch<RC, LM, MODE_F2, float, double, float>;
+#if TS_DIM != 3
+        ks->bf1 = launch_batched<RC, LM, MODE_F1, float, double, float>;
+        ks->bf2 = launch_batched<RC, LM, MODE_F2, float, double, float>;
+        ks->bs3 = launch_batched<RC, LM, MODE_S3, float, float, double>;
+        ks->bocc = batched_occupancy<RC, LM, float, float>;
+#endif
     }
 }
 #endif

@@ -197,30 +197,15 @@ template <int MODE> struct ModeTraits {
     static constexpr double cb = (MODE == MODE_S3) ? 2.0 / 3.0 : 0.25;
 };
 
-// ---- the 1-D / 2-D stage kernel -----------------------------------------------------------------
+// ---- one tile of the 1-D / 2-D stage: a strip of 60 columns marching over ty rows ----------------
+// block_min receives the tile's min(V/SR) (sr modes).  `tile` < 0 marks a wavefront without work
+// that still has to take part in a block reduction.
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
-__global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
-    static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
+__device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min) {
     using MT = ModeTraits<MODE>;
-    StepCtl* ctl = a.ctl;
-    __shared__ double red[4];
-    if constexpr (MT::sr) {
-        if (!ctl->active) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
-            return;
-        }
-    } else {
-        if (!ctl->go) return;
-    }
     const GridArgs& g = a.g;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nwaves = blockDim.x >> 6;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
-    const bool tile_ok = tile < a.ntiles;
-    if constexpr (!MT::sr) {
-        if (!tile_ok) return;  // no block-level synchronisation in these modes
-    }
+    const StepCtl* ctl = a.ctl;
+    const bool tile_ok = tile >= 0 && tile < a.ntiles;
     const int strip = tile_ok ? tile % a.nstrips : 0;
     const int yb = tile_ok ? tile / a.nstrips : 0;
 
@@ -290,7 +275,6 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     };
 
-    double block_min = __builtin_inf();
 
     if constexpr (DIM == 1) {
         // ---- one row ----
@@ -468,7 +452,113 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     }
 
-    if constexpr (MT::sr) dt_reduce_finalize(ctl, g.cfl, block_min, red, lane, wave, nwaves);
+}
+
+// ---- single-solve kernel: one wavefront per tile, block-level dt reduction ----------------------
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+__global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
+    static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
+    using MT = ModeTraits<MODE>;
+    StepCtl* ctl = a.ctl;
+    __shared__ double red[4];
+    if constexpr (MT::sr) {
+        if (!ctl->active) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
+            return;
+        }
+    } else {
+        if (!ctl->go) return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
+    if constexpr (!MT::sr) {
+        if (tile >= a.ntiles) return;  // no block-level synchronisation in these modes
+    }
+    double block_min = __builtin_inf();
+    stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, tile < a.ntiles ? tile : -1, lane, block_min);
+    if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
+}
+
+// ---- batched kernel: a persistent grid over a work-queue of (problem, tile) pairs ---------------
+// Every problem of the batch keeps its own KArgs (state, geometry, control block); wavefronts pull
+// tiles from an atomic counter until the queue is empty (every wavefront reaches that exit).  The
+// dt of each problem comes from a wavefront-level minimum and the last tile of that problem.  A
+// cell's arithmetic depends only on its own problem, so a result is the same alone or in a mix.
+struct BatchQueue {
+    int ntiles;          // tiles in the table
+    int pad;
+    unsigned counter[4]; // one work counter per kernel of a step
+};
+
+__device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, int lane, int ntiles) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+    if (lane == 0) {
+        atomicMin(&ctl->min_bits, dbits(m));
+        __threadfence();
+        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
+        if (prev == unsigned(ntiles - 1)) {
+            __threadfence();
+            double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
+            double dtn = mn * cfl;
+            double t0 = ctl->time;
+            if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
+            if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
+                ctl->error = 1;
+                ctl->go = 0;
+                ctl->active = 0;
+            } else {
+                double t1 = t0 + dtn;
+                ctl->dt = dtn;
+                ctl->time = t1;
+                ctl->iter = ctl->iter + 1;
+                int sd = ctl->steps_done + 1;
+                ctl->steps_done = sd;
+                ctl->go = 1;
+                bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
+                            (sd >= ctl->max_steps);
+                ctl->active = stop ? 0 : 1;
+            }
+            atomicExch(&ctl->min_bits, ~0ull);
+            atomicExch(&ctl->blocks_done, 0u);
+        }
+    }
+}
+
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+__global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict__ descs,
+                                                         const int2* __restrict__ table,
+                                                         BatchQueue* q, int slot, int nslots) {
+    using MT = ModeTraits<MODE>;
+    const int lane = threadIdx.x & 63;
+    // the previous kernel of the step has finished: its counter can be re-armed
+    if (blockIdx.x == 0 && threadIdx.x == 0) q->counter[(slot + nslots - 1) % nslots] = 0;
+    // The loop control must be wave-uniform scalars: the ticket is taken by lane 0 and moved to
+    // an SGPR with readfirstlane after the lanes reconverge.  (A ticket broadcast through a VGPR
+    // shuffle lets the compiler treat the exit as divergent and give lanes their own iterations.)
+    const int ntiles = __builtin_amdgcn_readfirstlane(q->ntiles);
+    for (int it = 0; it <= ntiles; it++) {  // at most one ticket per tile plus the final one
+        int tl = 0;
+        if (lane == 0) tl = int(atomicAdd(&q->counter[slot], 1u));
+        const int t = __builtin_amdgcn_readfirstlane(tl);
+        if (t >= ntiles) break;
+        const int2 e = table[t];
+        const KArgs& a = descs[e.x];
+        StepCtl* ctl = a.ctl;
+        if constexpr (MT::sr) {
+            if (!ctl->active) {
+                if (e.y == 0 && lane == 0) ctl->go = 0;
+                continue;
+            }
+        } else {
+            if (!ctl->go) continue;
+        }
+        double m = __builtin_inf();
+        stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, e.y, lane, m);
+        if constexpr (MT::sr) wave_reduce_finalize(ctl, a.g.cfl, m, lane, a.ntiles);
+    }
 }
 
 }  // namespace tsd

@@ -1,25 +1,643 @@
-// ts_batch.cpp -- batching and serving entry points (filled in by the batched engine).
+// ts_batch.cpp -- batching and serving: many independent solves advanced together on one GPU.
+//
+// Every resident solve keeps its own device state (U^n in double, L(U^n) and U^2 in the
+// stage-array type), geometry and control block inside one device arena.  A step of the batch is
+// the fused-dt SSP-RK3 sequence (three launches per kernel family) over a tile work-queue that
+// spans all resident solves of that family; each solve takes its own dt, stops at its own t_end
+// or step count, and its arithmetic does not depend on what else is in the batch.  Between steps
+// the engine retires finished solves (their state is copied back) and admits queued ones that
+// fit the HBM budget: continuous batching.  A background thread can drive it (serving).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
 
 #include "ts_internal.hpp"
 
-struct ts_batch {
-    std::string err = "batching is not available in this build";
+using namespace tsd;
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
+
+// First-fit free list over one device allocation.
+class Arena {
+   public:
+    size_t capacity = 0;
+    char* base = nullptr;
+    std::map<size_t, size_t> free_;  // offset -> size
+    size_t used = 0;
+    void init(char* b, size_t cap) {
+        base = b;
+        capacity = cap;
+        free_.clear();
+        free_[0] = cap;
+        used = 0;
+    }
+    bool alloc(size_t n, size_t* off) {
+        n = align_up(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                size_t rest = it->second - n;
+                size_t o = it->first;
+                free_.erase(it);
+                if (rest) free_[o + n] = rest;
+                used += n;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align_up(n);
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
 };
 
-extern "C" {
-int ts_batch_create(const ts_batch_config*, ts_batch** out) {
-    if (out) *out = nullptr;
-    return TS_ERR_UNSUPPORTED;
+using FamilyKey = std::tuple<int, int, int, int>;  // ndims, riemann, recon variant, precision
+
+struct Problem {
+    long long ticket = 0;
+    ts_config cfg{};
+    int max_steps = 0;
+    double t_end = 0;
+    std::vector<double> init;  // reference AoS state until admission
+    size_t bytes = 0, off = 0;
+    KArgs desc{};
+    int family = -1;
+    int ntiles = 0;
+    size_t ncell = 0;
+    int n[3] = {1, 1, 1};
+    std::vector<double> result;
+    ts_step_info info{};
+    int status = 0;  // 0 queued, 1 resident, 2 done
+    int error = 0;
+};
+
+struct Family {
+    FamilyKey key;
+    KernelSet ks;
+    int occupancy = 1;
+    // device tables
+    KArgs* d_desc = nullptr;
+    int2* d_table = nullptr;
+    BatchQueue* d_queue = nullptr;
+    size_t cap_desc = 0, cap_table = 0;
+    std::vector<long long> members;  // tickets of resident problems, in descriptor order
+    bool dirty = true;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+};
+
+}  // namespace
+
+struct ts_batch {
+    ts_batch_config bc{};
+    std::string err;
+    std::mutex mu;
+    std::condition_variable cv;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    char* d_arena = nullptr;
+    int ncu = 256;
+    long long next_ticket = 1;
+    std::deque<long long> queue;
+    std::unordered_map<long long, std::unique_ptr<Problem>> probs;
+    std::vector<Family> families;
+    std::vector<double> stage;  // host staging for state conversion
+    StepCtl* h_ctl = nullptr;   // pinned, one per resident problem
+    size_t h_ctl_cap = 0;
+    std::thread server;
+    std::atomic<bool> serving{false};
+    std::atomic<bool> stop{false};
+    int steps_per_chunk = 8;
+};
+
+namespace {
+
+int fail(ts_batch* b, int code, const std::string& m) {
+    b->err = m;
+    return code;
 }
+
+#define BCHK(call)                                      \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            b->err = tsr::hip_msg(e_, #call);           \
+            return TS_ERR_HIP;                          \
+        }                                               \
+    } while (0)
+
+size_t state_cells(const ts_config& c) { return (size_t)c.nx * c.ny * c.nz; }
+
+// geometry + state + control of one problem laid out in its arena block
+struct Layout {
+    size_t u0, l, u2, geo, ctl, markers, total;
+    size_t geo_doubles;
+};
+
+Layout layout_of(const ts_config& c) {
+    Layout L{};
+    size_t n = state_cells(c);
+    size_t se = c.precision == TS_FP32_RESIDUAL ? 4 : 8;
+    size_t geo = 0;
+    int na[3] = {c.nx, c.ny, c.nz};
+    for (int a = 0; a < 3; a++) {
+        int m = a < c.ndims ? na[a] : 1;
+        geo += 2 * (m + 2) + 3 * m;  // h, ih (with ghosts); rc, rl, rr (interior)
+    }
+    L.geo_doubles = geo;
+    size_t o = 0;
+    L.u0 = o; o = align_up(o + n * 5 * 8);
+    L.l = o; o = align_up(o + n * 5 * se);
+    L.u2 = o; o = align_up(o + n * 5 * se);
+    L.geo = o; o = align_up(o + geo * 8);
+    L.ctl = o; o = align_up(o + sizeof(StepCtl));
+    L.markers = o;
+    size_t mk = 0;
+    for (int s = 0; s < 6; s++)
+        if (s / 2 < c.ndims && c.face_marker[s]) mk += n / na[s / 2];
+    o = align_up(o + mk);
+    L.total = o;
+    return L;
+}
+
+int family_of(ts_batch* b, const ts_config& c) {
+    FamilyKey k(c.ndims, c.riemann, tsr::recon_variant(c), c.precision);
+    for (size_t i = 0; i < b->families.size(); i++)
+        if (b->families[i].key == k) return (int)i;
+    Family f;
+    f.key = k;
+    if (!tsr::lookup_kernels(c.ndims, c.riemann, tsr::recon_variant(c), c.precision, &f.ks) || !f.ks.bf1) return -1;
+    f.occupancy = f.ks.bocc ? f.ks.bocc() : 1;
+    b->families.push_back(f);
+    return (int)b->families.size() - 1;
+}
+
+// upload geometry/state/control of a problem into its arena block and fill its descriptor
+int admit(ts_batch* b, Problem& p) {
+    const ts_config& c = p.cfg;
+    Layout L = layout_of(c);
+    char* base = b->d_arena + p.off;
+    GridArgs& g = p.desc.g;
+    g = GridArgs{};
+    g.nd = c.ndims;
+    g.n[0] = c.nx; g.n[1] = c.ny; g.n[2] = c.nz;
+    for (int a = 0; a < 3; a++) g.periodic[a] = a < c.ndims ? c.periodic[a] : 1;
+    g.gamma = c.gamma;
+    g.eps = c.eps;
+    g.cfl = c.cfl;
+    std::vector<double> geo;
+    geo.reserve(L.geo_doubles);
+    double* dgeo = (double*)(base + L.geo);
+    bool uniform = true;
+    std::vector<double> X, H;
+    double h1[3] = {1.0, 1.0, 1.0};
+    for (int a = 0; a < 3; a++) {
+        int n = a < c.ndims ? p.n[a] : 1;
+        if (a < c.ndims) {
+            tsr::build_axis(c, a, X, H);
+            uniform = uniform && tsr::uniform_axis(H);
+        } else {
+            X.assign(3, 0.0);
+            H.assign(3, 1.0);
+        }
+        size_t oh = geo.size();
+        h1[a] = H[1];
+        for (int i = 0; i < n + 2; i++) geo.push_back(H[i]);
+        size_t oi = geo.size();
+        for (int i = 0; i < n + 2; i++) geo.push_back(1.0 / H[i]);
+        size_t orc = geo.size();
+        for (int i = 0; i < n; i++) geo.push_back(a < c.ndims ? 1.0 / (X[i + 2] - X[i]) : 0.0);
+        size_t orl = geo.size();
+        for (int i = 0; i < n; i++) geo.push_back(a < c.ndims ? 1.0 / (X[i + 1] - X[i]) : 0.0);
+        size_t orr = geo.size();
+        for (int i = 0; i < n; i++) geo.push_back(a < c.ndims ? 1.0 / (X[i + 2] - X[i + 1]) : 0.0);
+        g.h[a] = dgeo + oh;
+        g.ih[a] = dgeo + oi;
+        g.rc[a] = dgeo + orc;
+        g.rl[a] = dgeo + orl;
+        g.rr[a] = dgeo + orr;
+    }
+    g.eps2 = 0.0;
+    if (c.recon == TS_RECON_LINEAR_LIM && c.limiter == TS_LIMITER_VENKAT) {
+        if (!uniform) return fail(b, TS_ERR_UNSUPPORTED, "Venkatakrishnan limiter needs a uniform grid");
+        // limVenkatar::SendCellInfo (Limiters.h:227-230), K = 5
+        g.eps2 = std::pow(5.0 * std::pow(h1[0] * h1[1] * h1[2], 1.0 / c.ndims), 3.0);
+    }
+    for (int m = 0; m < TS_MAX_BC; m++) {
+        g.bc[m].type = m < c.n_bc ? c.bc[m].type : TS_BC_NATURAL;
+        for (int a = 0; a < 3; a++) g.bc[m].vel[a] = c.bc[m].velocity[a];
+        g.bc[m].p = c.bc[m].pstatic;
+        g.bc[m].rho = c.bc[m].density;
+    }
+    for (int s = 0; s < 6; s++) g.side_marker[s] = c.side_marker[s];
+    BCHK(hipMemcpyAsync(dgeo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, b->stream));
+    // state: AoS with ghosts -> SoA interior
+    size_t n = p.ncell;
+    if (b->stage.size() < n * 5) b->stage.resize(n * 5);
+    int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
+    size_t nxa = c.nx + 2, nya = c.ny + 2 * gy;
+    for (int k = 0; k < c.nz; k++)
+        for (int j = 0; j < c.ny; j++)
+            for (int i = 0; i < c.nx; i++) {
+                size_t src = (((size_t)(k + gz) * nya + (j + gy)) * nxa + (i + 1)) * 5;
+                size_t d = ((size_t)k * c.ny + j) * c.nx + i;
+                for (int v = 0; v < 5; v++) b->stage[v * n + d] = p.init[src + v];
+            }
+    BCHK(hipMemcpyAsync(base + L.u0, b->stage.data(), n * 5 * 8, hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));  // the staging buffer is reused
+    p.init.clear();
+    p.init.shrink_to_fit();
+    StepCtl ctl{};
+    ctl.t_max = p.t_end > 0 ? p.t_end : 1e300;
+    ctl.next_snap = p.t_end > 0 ? p.t_end : 0.0;
+    ctl.max_steps = p.max_steps;
+    ctl.active = p.max_steps > 0 ? 1 : 0;
+    ctl.min_bits = ~0ull;
+    // synchronous: the source lives on this stack frame
+    BCHK(hipMemcpy(base + L.ctl, &ctl, sizeof(StepCtl), hipMemcpyHostToDevice));
+    KArgs& d = p.desc;
+    d.uin = nullptr;
+    d.lin = nullptr;
+    d.un = (const double*)(base + L.u0);
+    d.uout = nullptr;
+    d.plane = n;
+    d.ctl = (StepCtl*)(base + L.ctl);
+    d.ty = c.ndims == 1 ? 1 : tsr::tuned_rows_per_tile(c);
+    d.nstrips = (c.nx + STRIP - 1) / STRIP;
+    d.ntiles = d.nstrips * (c.ndims == 1 ? 1 : (c.ny + d.ty - 1) / d.ty);
+    p.ntiles = d.ntiles;
+    p.status = 1;
+    return TS_OK;
+}
+
+// descriptor of problem p for one kernel of the step
+KArgs stage_desc(const Problem& p, int kernel, char* arena) {
+    Layout L = layout_of(p.cfg);
+    char* base = arena + p.off;
+    KArgs a = p.desc;
+    void* u0 = base + L.u0;
+    void* l = base + L.l;
+    void* u2 = base + L.u2;
+    if (kernel == 0) { a.uin = u0; a.uout = l; }
+    else if (kernel == 1) { a.uin = u0; a.lin = l; a.uout = u2; }
+    else { a.uin = u2; a.uout = u0; }
+    return a;
+}
+
+int rebuild_tables(ts_batch* b, Family& f) {
+    size_t np = f.members.size();
+    size_t nt = 0;
+    for (long long t : f.members) nt += b->probs[t]->ntiles;
+    bool realloc = false;
+    if (np > f.cap_desc) {  // cap_desc descriptors per kernel, three kernels
+        if (f.d_desc) BCHK(hipFree(f.d_desc));
+        f.cap_desc = std::max<size_t>(np * 2, 64);
+        BCHK(hipMalloc(&f.d_desc, 3 * f.cap_desc * sizeof(KArgs)));
+        realloc = true;
+    }
+    if (nt > f.cap_table) {
+        if (f.d_table) BCHK(hipFree(f.d_table));
+        f.cap_table = std::max<size_t>(nt * 2, 4096);
+        BCHK(hipMalloc(&f.d_table, f.cap_table * sizeof(int2)));
+        realloc = true;
+    }
+    if (!f.d_queue) {
+        BCHK(hipMalloc(&f.d_queue, sizeof(BatchQueue)));
+        BatchQueue z{};
+        BCHK(hipMemcpy(f.d_queue, &z, sizeof(z), hipMemcpyHostToDevice));  // source on the stack
+        realloc = true;
+    }
+    if (realloc && f.graph) {
+        hipGraphExecDestroy(f.graph);
+        f.graph = nullptr;
+    }
+    // descriptors: one per (kernel, problem), kernel k's block starting at k * cap_desc so that
+    // the launch arguments stay fixed while the resident set changes
+    std::vector<KArgs> desc(3 * f.cap_desc);
+    std::vector<int2> table;
+    table.reserve(nt);
+    for (size_t i = 0; i < np; i++) {
+        const Problem& p = *b->probs[f.members[i]];
+        for (int k = 0; k < 3; k++) desc[k * f.cap_desc + i] = stage_desc(p, k, b->d_arena);
+        for (int t = 0; t < p.ntiles; t++) table.push_back(make_int2((int)i, t));
+    }
+    BCHK(hipMemcpyAsync(f.d_desc, desc.data(), desc.size() * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
+    if (nt) BCHK(hipMemcpyAsync(f.d_table, table.data(), nt * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+    // queue header: tile count (the counters are re-armed by the kernels themselves)
+    BCHK(hipStreamSynchronize(b->stream));  // descriptor / table sources are local vectors
+    int ntiles = (int)nt;
+    BCHK(hipMemcpy(&f.d_queue->ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice));
+    f.dirty = false;
+    return TS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// launches of one batched step for family f
+void issue_step(ts_batch* b, Family& f) {
+    int nblk = b->ncu * f.occupancy;
+    f.ks.bf1(f.d_desc, f.d_table, f.d_queue, 0, 3, nblk, b->stream);
+    f.ks.bf2(f.d_desc + f.cap_desc, f.d_table, f.d_queue, 1, 3, nblk, b->stream);
+    f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_queue, 2, 3, nblk, b->stream);
+}
+
+int issue_steps(ts_batch* b, Family& f, int n) {
+    if (!b->bc.use_graphs) {
+        for (int s = 0; s < n; s++) issue_step(b, f);
+        BCHK(hipGetLastError());
+        return TS_OK;
+    }
+    if (!f.graph || f.graph_steps != n) {
+        if (f.graph) BCHK(hipGraphExecDestroy(f.graph));
+        f.graph = nullptr;
+        hipGraph_t gr;
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < n; s++) issue_step(b, f);
+        BCHK(hipStreamEndCapture(b->stream, &gr));
+        hipError_t e = hipGraphInstantiate(&f.graph, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            f.graph = nullptr;
+            return fail(b, TS_ERR_HIP, tsr::hip_msg(e, "hipGraphInstantiate"));
+        }
+        f.graph_steps = n;
+    }
+    BCHK(hipGraphLaunch(f.graph, b->stream));
+    return TS_OK;
+}
+
+// copy a finished problem's state back to the host (reference AoS layout, ghosts as images)
+int retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
+    const ts_config& c = p.cfg;
+    Layout L = layout_of(c);
+    size_t n = p.ncell;
+    if (b->stage.size() < n * 5) b->stage.resize(n * 5);
+    BCHK(hipMemcpyAsync(b->stage.data(), b->d_arena + p.off + L.u0, n * 5 * 8, hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
+    size_t nxa = c.nx + 2, nya = c.ny + 2 * gy, nza = c.nz + 2 * gz;
+    p.result.assign(nxa * nya * nza * 5, 0.0);
+    for (int k = 0; k < c.nz; k++)
+        for (int j = 0; j < c.ny; j++)
+            for (int i = 0; i < c.nx; i++) {
+                size_t dst = (((size_t)(k + gz) * nya + (j + gy)) * nxa + (i + 1)) * 5;
+                size_t s = ((size_t)k * c.ny + j) * c.nx + i;
+                for (int v = 0; v < 5; v++) p.result[dst + v] = b->stage[v * n + s];
+            }
+    p.info.time = ctl.time;
+    p.info.dt = ctl.dt;
+    p.info.iteration = ctl.iter;
+    p.info.steps_done = ctl.steps_done;
+    for (int v = 0; v < 5; v++) p.info.residual_norm[v] = 0;
+    p.error = ctl.error;
+    p.status = 2;
+    b->arena.release(p.off, p.bytes);
+    return TS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out) {
+    if (!out) return TS_ERR_INVALID;
+    *out = nullptr;
+    ts_batch* b = new ts_batch();
+    if (bcfg) b->bc = *bcfg;
+    if (b->bc.hbm_budget_bytes == 0) b->bc.hbm_budget_bytes = (size_t)8 << 30;
+    if (b->bc.max_active <= 0) b->bc.max_active = 1 << 20;
+    hipError_t e;
+    if ((e = hipSetDevice(b->bc.device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_arena, b->bc.hbm_budget_bytes)) != hipSuccess) {
+        fprintf(stderr, "ts_batch_create: %s\n", hipGetErrorString(e));
+        if (b->stream) hipStreamDestroy(b->stream);
+        delete b;
+        return e == hipErrorOutOfMemory ? TS_ERR_NOMEM : TS_ERR_HIP;
+    }
+    b->arena.init(b->d_arena, b->bc.hbm_budget_bytes);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->bc.device) == hipSuccess) b->ncu = prop.multiProcessorCount;
+    *out = b;
+    return TS_OK;
+}
+
 int ts_batch_destroy(ts_batch* b) {
+    if (!b) return TS_ERR_INVALID;
+    ts_batch_stop_server(b);
+    hipSetDevice(b->bc.device);
+    hipStreamSynchronize(b->stream);
+    for (Family& f : b->families) {
+        if (f.graph) hipGraphExecDestroy(f.graph);
+        if (f.d_desc) hipFree(f.d_desc);
+        if (f.d_table) hipFree(f.d_table);
+        if (f.d_queue) hipFree(f.d_queue);
+    }
+    if (b->h_ctl) hipHostFree(b->h_ctl);
+    if (b->d_arena) hipFree(b->d_arena);
+    if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return TS_OK;
 }
+
 const char* ts_batch_last_error(const ts_batch* b) { return b ? b->err.c_str() : "null batch"; }
-int ts_batch_submit(ts_batch*, const ts_config*, const double*, int, double, long long*) { return TS_ERR_UNSUPPORTED; }
-int ts_batch_run(ts_batch*, int, int*) { return TS_ERR_UNSUPPORTED; }
-int ts_batch_poll(ts_batch*, long long, int*, double*, ts_step_info*) { return TS_ERR_UNSUPPORTED; }
-int ts_batch_start_server(ts_batch*) { return TS_ERR_UNSUPPORTED; }
-int ts_batch_stop_server(ts_batch*) { return TS_ERR_UNSUPPORTED; }
+
+int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state, int max_steps,
+                    double t_end, long long* ticket) {
+    if (!b || !cfg || !init_state || !ticket || max_steps < 0) return TS_ERR_INVALID;
+    std::string err;
+    // the same validation as ts_create, then the batch's own limits
+    ts_ctx* probe = nullptr;
+    (void)probe;
+    if (cfg->ndims < 1 || cfg->ndims > 2) return fail(b, TS_ERR_UNSUPPORTED, "batched solves are 1-D or 2-D");
+    if (cfg->rk_stages != 3) return fail(b, TS_ERR_UNSUPPORTED, "batched solves use SSP-RK3");
+    if (cfg->ghost != 1 || cfg->nx < 2 || (cfg->ndims > 1 && cfg->ny < 2) || cfg->nz != 1 ||
+        (cfg->ndims == 1 && cfg->ny != 1) || !(cfg->gamma > 1.0) || !(cfg->cfl > 0.0) || cfg->n_bc < 1 ||
+        cfg->n_bc > TS_MAX_BC)
+        return fail(b, TS_ERR_INVALID, "invalid configuration");
+    for (int s = 0; s < 6; s++)
+        if (cfg->face_marker[s]) return fail(b, TS_ERR_UNSUPPORTED, "per-face markers are not batched");
+    for (int s = 0; s < 2 * cfg->ndims; s++)
+        if (!cfg->periodic[s / 2] && (cfg->side_marker[s] < 0 || cfg->side_marker[s] >= cfg->n_bc))
+            return fail(b, TS_ERR_INVALID, "side marker refers to a missing boundary condition");
+    auto p = std::make_unique<Problem>();
+    p->cfg = *cfg;
+    for (int a = 0; a < 3; a++) { p->cfg.coord[a] = nullptr; p->cfg.h[a] = nullptr; }
+    p->n[0] = cfg->nx;
+    p->n[1] = cfg->ny;
+    p->n[2] = cfg->nz;
+    p->ncell = state_cells(*cfg);
+    p->max_steps = max_steps;
+    p->t_end = t_end;
+    p->bytes = layout_of(*cfg).total;
+    if (p->bytes > b->arena.capacity) return fail(b, TS_ERR_NOMEM, "solve is larger than the HBM budget");
+    size_t nst = ts_state_size_cfg(cfg);
+    p->init.assign(init_state, init_state + nst);
+    std::lock_guard<std::mutex> lk(b->mu);
+    int fam = family_of(b, *cfg);
+    if (fam < 0) return fail(b, TS_ERR_UNSUPPORTED, "no batched kernel for this configuration");
+    p->family = fam;
+    p->ticket = b->next_ticket++;
+    *ticket = p->ticket;
+    b->queue.push_back(p->ticket);
+    b->probs[p->ticket] = std::move(p);
+    b->cv.notify_all();
+    return TS_OK;
 }
+
+int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
+    if (!b || n_steps < 0) return TS_ERR_INVALID;
+    BCHK(hipSetDevice(b->bc.device));
+    std::vector<Problem*> resident;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        // admit queued solves in arrival order while they fit the budget
+        size_t nres = 0;
+        for (Family& f : b->families) nres += f.members.size();
+        while (!b->queue.empty() && (int)nres < b->bc.max_active) {
+            Problem& p = *b->probs[b->queue.front()];
+            size_t off;
+            if (!b->arena.alloc(p.bytes, &off)) break;  // head-of-line waits for memory
+            p.off = off;
+            int r = admit(b, p);
+            if (r) {
+                b->arena.release(off, p.bytes);
+                p.status = 2;
+                p.error = 1;
+                b->queue.pop_front();
+                continue;
+            }
+            b->families[p.family].members.push_back(p.ticket);
+            b->families[p.family].dirty = true;
+            b->queue.pop_front();
+            nres++;
+        }
+        for (Family& f : b->families)
+            for (long long t : f.members) resident.push_back(b->probs[t].get());
+    }
+    if (!resident.empty() && n_steps > 0) {
+        for (Family& f : b->families) {
+            if (f.members.empty()) continue;
+            if (f.dirty) {
+                int r = rebuild_tables(b, f);
+                if (r) return r;
+            }
+            int r = issue_steps(b, f, n_steps);
+            if (r) return r;
+        }
+        // control blocks of every resident solve back to the host
+        if (b->h_ctl_cap < resident.size()) {
+            if (b->h_ctl) BCHK(hipHostFree(b->h_ctl));
+            b->h_ctl_cap = resident.size() * 2;
+            BCHK(hipHostMalloc((void**)&b->h_ctl, b->h_ctl_cap * sizeof(StepCtl), hipHostMallocDefault));
+        }
+        for (size_t i = 0; i < resident.size(); i++)
+            BCHK(hipMemcpyAsync(&b->h_ctl[i], resident[i]->desc.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
+        BCHK(hipStreamSynchronize(b->stream));
+        std::lock_guard<std::mutex> lk(b->mu);
+        for (size_t i = 0; i < resident.size(); i++) {
+            Problem& p = *resident[i];
+            const StepCtl& c = b->h_ctl[i];
+            if (c.active && !c.error) continue;
+            int r = retire(b, p, c);
+            if (r) return r;
+            Family& f = b->families[p.family];
+            f.members.erase(std::find(f.members.begin(), f.members.end(), p.ticket));
+            f.dirty = true;
+        }
+    }
+    if (pending) {
+        std::lock_guard<std::mutex> lk(b->mu);
+        size_t n = b->queue.size();
+        for (Family& f : b->families) n += f.members.size();
+        *pending = (int)n;
+    }
+    b->cv.notify_all();
+    return TS_OK;
+}
+
+int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state, ts_step_info* info) {
+    if (!b || !done) return TS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(b->mu);
+    auto it = b->probs.find(ticket);
+    if (it == b->probs.end()) return fail(b, TS_ERR_INVALID, "unknown ticket");
+    Problem& p = *it->second;
+    if (p.status != 2) {
+        *done = 0;
+        return TS_OK;
+    }
+    *done = 1;
+    if (out_state) std::memcpy(out_state, p.result.data(), p.result.size() * sizeof(double));
+    if (info) *info = p.info;
+    int err = p.error;
+    b->probs.erase(it);
+    return err ? TS_ERR_UNPHYSICAL : TS_OK;
+}
+
+int ts_batch_start_server(ts_batch* b) {
+    if (!b) return TS_ERR_INVALID;
+    if (b->serving.exchange(true)) return TS_OK;
+    b->stop = false;
+    b->server = std::thread([b] {
+        while (!b->stop) {
+            {
+                std::unique_lock<std::mutex> lk(b->mu);
+                bool work = !b->queue.empty();
+                for (Family& f : b->families) work = work || !f.members.empty();
+                if (!work) {
+                    b->cv.wait_for(lk, std::chrono::milliseconds(2));
+                    continue;
+                }
+            }
+            int pending = 0;
+            if (ts_batch_run(b, b->steps_per_chunk, &pending) != TS_OK) {
+                std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            }
+        }
+    });
+    return TS_OK;
+}
+
+int ts_batch_stop_server(ts_batch* b) {
+    if (!b) return TS_ERR_INVALID;
+    if (!b->serving.exchange(false)) return TS_OK;
+    b->stop = true;
+    b->cv.notify_all();
+    if (b->server.joinable()) b->server.join();
+    return TS_OK;
+}
+
+}  // extern "C"

@@ -92,6 +92,129 @@ def cpu_baseline(budget_s=15.0):
                        f"{threads} std::threads over grid lines")
 
 
+def batched_tubes(count=4096, sample=48, seed=2024):
+    """Batched form: `count` independent shock tubes (exact Godunov with FP64 Newton, FP32 bulk,
+    SSP-RK3) in one persistent work-queue launch per stage, against one-at-a-time FP64 solves of
+    a random sample of the same tubes (cell-updates/s of each), with density L1 error."""
+    from turbostructred_amd import workloads
+    tubes = workloads.shock_tubes(count, seed=seed, precision=capi.FP32_RESIDUAL)
+    b = capi.Batch(hbm_budget_bytes=4 << 30)
+    # warm-up: kernel loading and graph capture on a few tubes
+    w = [b.submit(cfg, U, 20, te) for cfg, U, te in tubes[:8]]
+    while w:
+        b.run(8)
+        w = [t for t in w if not b.poll(t)[0]]
+    t0 = time.perf_counter()
+    tickets = [b.submit(cfg, U, 10**6, te) for cfg, U, te in tubes]
+    while b.run(32) > 0:
+        pass
+    t_batch = time.perf_counter() - t0
+    results = {}
+    for t in tickets:
+        done, st, info = b.poll(t)
+        assert done
+        results[t] = (st, info)
+    cu_batch = sum(cfg.nx * results[t][1].steps_done for (cfg, _, _), t in zip(tubes, tickets))
+    b.close()
+    # one-at-a-time FP64 solves of a sample
+    rng = np.random.default_rng(seed + 1)
+    idx = rng.choice(count, size=sample, replace=False)
+    cu_single, t_single, l1 = 0, 0.0, []
+    for i in idx:
+        cfg32, U, te = tubes[i]
+        cfg = capi.make_config(ndims=1, nx=cfg32.nx, cfl=0.45, gamma=cfg32.gamma, riemann=capi.RIEMANN_GODUNOV,
+                               recon=capi.RECON_LINEAR_LIM, rk_stages=3, precision=capi.FP64)
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        t1 = time.perf_counter()
+        info = s.step(10**6, te, te)
+        t_single += time.perf_counter() - t1
+        cu_single += cfg.nx * info.steps_done
+        ref = capi.interior(cfg, s.get_state())[..., 0]
+        got = capi.interior(cfg32, results[tickets[i]][0])[..., 0]
+        l1.append(float(np.abs(got - ref).mean()))
+    rate_b, rate_s = cu_batch / t_batch, cu_single / t_single
+    return {"workload": f"{count} shock tubes, N~logU[64,4096] ({sum(c.nx for c, _, _ in tubes)} cells), "
+                        "exact Godunov (FP64 Newton) + limited linear, SSP-RK3, FP32 residual, t_end in-domain",
+            "cell_updates_per_s": rate_b, "seconds": t_batch,
+            "fp64_one_at_a_time_cell_updates_per_s": rate_s, "fp64_sample_tubes": sample,
+            "speedup_vs_fp64_one_at_a_time": rate_b / rate_s,
+            "density_l1_vs_fp64_max": max(l1), "density_l1_vs_fp64_mean": float(np.mean(l1))}
+
+
+def serving(rates=(50, 100, 200, 500, 1000, 2000), requests=48, seed=7):
+    """Served form: 2-D Riemann problems arrive as a Poisson process; a background engine thread
+    admits them by HBM footprint and batches them continuously (FP32 residual, FP64 state).
+    Latency = arrival -> result on the host."""
+    import threading
+    from turbostructred_amd import workloads
+    out = {"workload": "2-D four-quadrant Riemann problems, L~logU[64,1024], steps~U{50..500}, "
+                       "HLLC + limited linear + SSP-RK3, FP32 residual / FP64 state", "rates": []}
+    reqs = workloads.riemann_2d(requests, seed=seed, precision=capi.FP32_RESIDUAL)
+    work = sum(c.nx * c.ny * k for c, _, k in reqs)
+    b = capi.Batch(hbm_budget_bytes=16 << 30)
+    w = [b.submit(cfg, U, 5) for cfg, U, _ in reqs[:4]]  # warm-up
+    while w:
+        b.run(5)
+        w = [t for t in w if not b.poll(t)[0]]
+    b.start_server()
+    served = {}
+    for lam in rates:
+        rng = np.random.default_rng(seed + lam)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+        lat = [None] * requests
+        tick = [None] * requests
+        t0 = time.perf_counter()
+        done_count = [0]
+
+        def submitter():
+            for i, (cfg, U, k) in enumerate(reqs):
+                wait = t0 + arrivals[i] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                tick[i] = b.submit(cfg, U, k)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        while done_count[0] < requests:
+            for i in range(requests):
+                if lat[i] is None and tick[i] is not None:
+                    d, st, _ = b.poll(tick[i])
+                    if d:
+                        lat[i] = time.perf_counter() - (t0 + arrivals[i])
+                        done_count[0] += 1
+                        if lam == rates[0]:
+                            served[i] = st
+            time.sleep(0.0005)
+        th.join()
+        span = time.perf_counter() - t0
+        la = np.array(lat) * 1e3
+        out["rates"].append({"rate_per_s": lam, "p50_ms": float(np.percentile(la, 50)),
+                             "p99_ms": float(np.percentile(la, 99)), "throughput_cell_updates_per_s": work / span,
+                             "requests": requests})
+    b.stop_server()
+    b.close()
+    # accuracy: FP32-residual served results against FP64 solves of the smallest requests
+    small = sorted(range(requests), key=lambda i: reqs[i][0].nx)[:4]
+    l1 = []
+    t_fp64, cu_fp64 = 0.0, 0
+    for i in small:
+        cfg32, U, k = reqs[i]
+        cfg = capi.make_config(ndims=2, nx=cfg32.nx, ny=cfg32.ny, cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                               recon=capi.RECON_LINEAR_LIM, rk_stages=3, precision=capi.FP64)
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        t1 = time.perf_counter()
+        s.step(k)
+        t_fp64 += time.perf_counter() - t1
+        cu_fp64 += cfg.nx * cfg.ny * k
+        ref = capi.interior(cfg, s.get_state())[..., 0]
+        l1.append(float(np.abs(capi.interior(cfg32, served[i])[..., 0] - ref).mean()))
+    out["density_l1_vs_fp64_max"] = max(l1)
+    out["fp64_one_at_a_time_cell_updates_per_s_small"] = cu_fp64 / t_fp64
+    return out
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
     path = os.path.join(ROOT, "profiles", "c2_2048_fp64_hbm_traffic.md")
@@ -115,6 +238,8 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -153,6 +278,10 @@ def main():
         "final_time": info.time,
         "finite": bool(np.isfinite(state).all()),
     }
+    if not args.no_batch:
+        out["batched"] = batched_tubes()
+    if not args.no_serve:
+        out["serving"] = serving()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

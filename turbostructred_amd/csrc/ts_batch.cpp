@@ -32,6 +32,18 @@ using namespace tsd;
 namespace {
 
 constexpr size_t ALIGN = 256;
+
+static_assert(sizeof(StepCtl) % 8 == 0, "control block is copied as 8-byte words");
+
+// One launch packs the control blocks of every resident solve (scattered over the arena) into a
+// contiguous buffer, so that they come back to the host in a single copy.
+__global__ void __launch_bounds__(256) gather_ctl(StepCtl* const* src, StepCtl* dst, int n) {
+    constexpr int W = sizeof(StepCtl) / 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * W) return;
+    const int p = i / W, w = i - p * W;
+    reinterpret_cast<unsigned long long*>(dst + p)[w] = reinterpret_cast<const unsigned long long*>(src[p])[w];
+}
 inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 
 // First-fit free list over one device allocation.
@@ -134,6 +146,9 @@ struct ts_batch {
     std::vector<Family> families;
     std::vector<double> stage;  // host staging for state conversion
     StepCtl* h_ctl = nullptr;   // pinned, one per resident problem
+    StepCtl** h_ptr = nullptr;  // pinned, device address of each resident problem's control block
+    StepCtl* d_ctl = nullptr;   // packed copies of the resident control blocks
+    StepCtl** d_ptr = nullptr;
     size_t h_ctl_cap = 0;
     std::thread server;
     std::atomic<bool> serving{false};
@@ -399,14 +414,19 @@ int issue_steps(ts_batch* b, Family& f, int n) {
     return TS_OK;
 }
 
-// copy a finished problem's state back to the host (reference AoS layout, ghosts as images)
-int retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
+// copy a finished problem's state back to the host: the copy is issued here (into p.init, which
+// is no longer needed after admission) and finish_retire converts it once the stream has synced
+int start_retire(ts_batch* b, Problem& p) {
+    Layout L = layout_of(p.cfg);
+    p.init.resize(p.ncell * 5);
+    BCHK(hipMemcpyAsync(p.init.data(), b->d_arena + p.off + L.u0, p.ncell * 5 * 8, hipMemcpyDeviceToHost, b->stream));
+    return TS_OK;
+}
+
+// SoA interior -> reference AoS layout (ghosts zero), control fields -> step info; frees the block
+void finish_retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
     const ts_config& c = p.cfg;
-    Layout L = layout_of(c);
-    size_t n = p.ncell;
-    if (b->stage.size() < n * 5) b->stage.resize(n * 5);
-    BCHK(hipMemcpyAsync(b->stage.data(), b->d_arena + p.off + L.u0, n * 5 * 8, hipMemcpyDeviceToHost, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));
+    const size_t n = p.ncell;
     int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
     size_t nxa = c.nx + 2, nya = c.ny + 2 * gy, nza = c.nz + 2 * gz;
     p.result.assign(nxa * nya * nza * 5, 0.0);
@@ -415,8 +435,9 @@ int retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
             for (int i = 0; i < c.nx; i++) {
                 size_t dst = (((size_t)(k + gz) * nya + (j + gy)) * nxa + (i + 1)) * 5;
                 size_t s = ((size_t)k * c.ny + j) * c.nx + i;
-                for (int v = 0; v < 5; v++) p.result[dst + v] = b->stage[v * n + s];
+                for (int v = 0; v < 5; v++) p.result[dst + v] = p.init[v * n + s];
             }
+    std::vector<double>().swap(p.init);
     p.info.time = ctl.time;
     p.info.dt = ctl.dt;
     p.info.iteration = ctl.iter;
@@ -425,7 +446,6 @@ int retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
     p.error = ctl.error;
     p.status = 2;
     b->arena.release(p.off, p.bytes);
-    return TS_OK;
 }
 
 }  // namespace
@@ -467,6 +487,9 @@ int ts_batch_destroy(ts_batch* b) {
         if (f.d_queue) hipFree(f.d_queue);
     }
     if (b->h_ctl) hipHostFree(b->h_ctl);
+    if (b->h_ptr) hipHostFree(b->h_ptr);
+    if (b->d_ctl) hipFree(b->d_ctl);
+    if (b->d_ptr) hipFree(b->d_ptr);
     if (b->d_arena) hipFree(b->d_arena);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -560,20 +583,41 @@ int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
         }
         // control blocks of every resident solve back to the host
         if (b->h_ctl_cap < resident.size()) {
+            BCHK(hipStreamSynchronize(b->stream));
             if (b->h_ctl) BCHK(hipHostFree(b->h_ctl));
-            b->h_ctl_cap = resident.size() * 2;
-            BCHK(hipHostMalloc((void**)&b->h_ctl, b->h_ctl_cap * sizeof(StepCtl), hipHostMallocDefault));
+            if (b->h_ptr) BCHK(hipHostFree(b->h_ptr));
+            if (b->d_ctl) BCHK(hipFree(b->d_ctl));
+            if (b->d_ptr) BCHK(hipFree(b->d_ptr));
+            b->h_ctl = nullptr, b->h_ptr = nullptr, b->d_ctl = nullptr, b->d_ptr = nullptr;
+            b->h_ctl_cap = 0;
+            const size_t cap = resident.size() * 2;
+            BCHK(hipHostMalloc((void**)&b->h_ctl, cap * sizeof(StepCtl), hipHostMallocDefault));
+            BCHK(hipHostMalloc((void**)&b->h_ptr, cap * sizeof(StepCtl*), hipHostMallocDefault));
+            BCHK(hipMalloc((void**)&b->d_ctl, cap * sizeof(StepCtl)));
+            BCHK(hipMalloc((void**)&b->d_ptr, cap * sizeof(StepCtl*)));
+            b->h_ctl_cap = cap;
         }
-        for (size_t i = 0; i < resident.size(); i++)
-            BCHK(hipMemcpyAsync(&b->h_ctl[i], resident[i]->desc.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
+        const int nres = (int)resident.size();
+        for (int i = 0; i < nres; i++) b->h_ptr[i] = resident[i]->desc.ctl;
+        BCHK(hipMemcpyAsync(b->d_ptr, b->h_ptr, nres * sizeof(StepCtl*), hipMemcpyHostToDevice, b->stream));
+        const int words = nres * int(sizeof(StepCtl) / 8);
+        hipLaunchKernelGGL(gather_ctl, dim3((words + 255) / 256), dim3(256), 0, b->stream, b->d_ptr, b->d_ctl, nres);
+        BCHK(hipGetLastError());
+        BCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, nres * sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
         BCHK(hipStreamSynchronize(b->stream));
-        std::lock_guard<std::mutex> lk(b->mu);
-        for (size_t i = 0; i < resident.size(); i++) {
-            Problem& p = *resident[i];
+        std::vector<int> fin;
+        for (int i = 0; i < nres; i++) {
             const StepCtl& c = b->h_ctl[i];
             if (c.active && !c.error) continue;
-            int r = retire(b, p, c);
+            int r = start_retire(b, *resident[i]);
             if (r) return r;
+            fin.push_back(i);
+        }
+        if (!fin.empty()) BCHK(hipStreamSynchronize(b->stream));
+        std::lock_guard<std::mutex> lk(b->mu);
+        for (int i : fin) {
+            Problem& p = *resident[i];
+            finish_retire(b, p, b->h_ctl[i]);
             Family& f = b->families[p.family];
             f.members.erase(std::find(f.members.begin(), f.members.end(), p.ticket));
             f.dirty = true;

@@ -364,6 +364,11 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
 }
 
 // ---- exact Riemann solver (always double) -----------------------------------------------------
+// Every power in the exact solver goes through one out-of-line copy of ocml's double pow: inlined
+// at each of its dozen call sites it made the kernels larger than the instruction cache, and
+// the run time was then set by instruction fetch.
+static __device__ __noinline__ double dpow(double x, double y) { return pow(x, y); }
+
 // f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37).
 __device__ inline double fk_dev(double g, double r, double pk, double ak, double p, double& d) {
     if (p > pk) {
@@ -374,9 +379,9 @@ __device__ inline double fk_dev(double g, double r, double pk, double ak, double
         return (p - pk) * q;
     }
     double pr = p / pk;
-    double res = pow(pr, (g - 1.0) / (2.0 * g)) - 1.0;
+    double res = dpow(pr, (g - 1.0) / (2.0 * g)) - 1.0;
     res *= 2.0 * ak / (g - 1.0);
-    d = pow(pr, -(g + 1.0) / (2.0 * g)) / (r * ak);
+    d = dpow(pr, -(g + 1.0) / (2.0 * g)) / (r * ak);
     return res;
 }
 
@@ -395,7 +400,14 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
     double du = ur - ul;
     s.vacuum = (2.0 / (g - 1.0)) * (al + ar) <= du;
     double p = 0.0;
-    if (!s.vacuum) {
+    // Inputs that are not a physical state (zero, negative or NaN density or pressure) give a
+    // NaN star pressure straight away.  Such inputs reach the solver on halo lanes, which are
+    // fed by a cross-lane shift and whose flux is discarded; iterating on them to the cap would
+    // hold the whole wavefront.  (For a face whose flux is used they give NaN either way.)
+    const bool physical = rl > 0.0 && pl > 0.0 && rr > 0.0 && pr > 0.0;
+    if (!physical) {
+        p = __builtin_nan("");
+    } else if (!s.vacuum) {
         double pmin = __builtin_fmin(pl, pr), pmax = __builtin_fmax(pl, pr);
         double ppv = 0.5 * (pl + pr) - 0.125 * du * (rl + rr) * (al + ar);
         ppv = __builtin_fmax(0.0, ppv);
@@ -403,9 +415,9 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
             p = ppv;
         } else if (ppv < pmin) {
             double z = (g - 1.0) / (2.0 * g);
-            double plz = pow(pl, z), prz = pow(pr, z);
+            double plz = dpow(pl, z), prz = dpow(pr, z);
             double t = (al + ar - 0.5 * (g - 1.0) * du) / (al / plz + ar / prz);
-            p = pow(t, 1.0 / z);
+            p = dpow(t, 1.0 / z);
         } else {
             double A = 2.0 / ((g + 1.0) * rl), B = (g - 1.0) / (g + 1.0) * pl;
             double gl = __builtin_sqrt(A / (ppv + B));
@@ -440,9 +452,9 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SL));
     } else {
         s.shock_l = false;
-        s.rl = rl * pow(prl, 1.0 / g);
+        s.rl = rl * dpow(prl, 1.0 / g);
         s.SHL = ul - al;
-        s.STL = s.u - al * pow(prl, (g - 1) / (2 * g));
+        s.STL = s.u - al * dpow(prl, (g - 1) / (2 * g));
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHL));
     }
     double prr = p / pr;
@@ -453,9 +465,9 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SR));
     } else {
         s.shock_r = false;
-        s.rr = rr * pow(prr, 1.0 / g);
+        s.rr = rr * dpow(prr, 1.0 / g);
         s.SHR = ur + ar;
-        s.STR = s.u + ar * pow(prr, (g - 1) / (2 * g));
+        s.STR = s.u + ar * dpow(prr, (g - 1) / (2 * g));
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHR));
     }
     if (s.vacuum) {  // the fan tails bound the vacuum (Toro sec. 4.6)
@@ -495,9 +507,9 @@ __device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)
         } else {
             double al = __builtin_sqrt(g * pL / UL[0]);
             double CL = 2.0 / (g + 1) + (g - 1) * (uL - S) / ((g + 1) * al);
-            ro = pow(CL, 2.0 / (g - 1)) * UL[0];
+            ro = dpow(CL, 2.0 / (g - 1)) * UL[0];
             u = (al + 0.5 * (g - 1) * uL + S) * (2 / (g + 1));
-            p = pow(CL, 2.0 * g / (g - 1)) * pL;
+            p = dpow(CL, 2.0 * g / (g - 1)) * pL;
         }
     } else {
         if (st.shock_r && !st.vacuum) {
@@ -509,9 +521,9 @@ __device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)
         } else {
             double ar = __builtin_sqrt(g * pR / UR[0]);
             double CR = 2.0 / (g + 1) - (g - 1) * (uR - S) / ((g + 1) * ar);
-            ro = pow(CR, 2.0 / (g - 1)) * UR[0];
+            ro = dpow(CR, 2.0 / (g - 1)) * UR[0];
             u = (-ar + 0.5 * (g - 1) * uR + S) * (2 / (g + 1));
-            p = pow(CR, 2.0 * g / (g - 1)) * pR;
+            p = dpow(CR, 2.0 * g / (g - 1)) * pR;
         }
     }
     const double* vel = left ? vL : vR;

@@ -7,9 +7,9 @@
 namespace tsd {
 
 using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
-// batched (persistent, work-queue) launch: descriptors, tile table, queue, counter slot, slots,
-// number of 256-thread blocks
-using BatchLaunchFn = void (*)(const KArgs*, const int2*, BatchQueue*, int, int, int, hipStream_t);
+// batched (persistent) launch: descriptors, (problem, tile) table, device tile count, number of
+// 256-thread blocks
+using BatchLaunchFn = void (*)(const KArgs*, const int2*, const int*, int, hipStream_t);
 using OccupancyFn = int (*)();  // resident 256-thread blocks per CU of the batched kernels
 
 // dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3, and the

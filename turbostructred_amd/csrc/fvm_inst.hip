@@ -32,10 +32,9 @@ static void launch(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
 
 #if TS_DIM != 3
 template <int RC, int LM, int MODE, typename R, typename TIn, typename TOut>
-static void launch_batched(const KArgs* d, const int2* t, BatchQueue* q, int slot, int nslots, int nblk,
-                           hipStream_t s) {
+static void launch_batched(const KArgs* d, const int2* t, const int* ntiles, int nblk, hipStream_t s) {
     hipLaunchKernelGGL((fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE, R, TIn, TOut>), dim3(nblk), dim3(256), 0, s,
-                       d, t, q, slot, nslots);
+                       d, t, ntiles);
 }
 template <int RC, int LM, typename R, typename S>
 static int batched_occupancy() {

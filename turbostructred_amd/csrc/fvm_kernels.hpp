@@ -481,17 +481,14 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
 }
 
-// ---- batched kernel: a persistent grid over a work-queue of (problem, tile) pairs ---------------
-// Every problem of the batch keeps its own KArgs (state, geometry, control block); wavefronts pull
-// tiles from an atomic counter until the queue is empty (every wavefront reaches that exit).  The
-// dt of each problem comes from a wavefront-level minimum and the last tile of that problem.  A
-// cell's arithmetic depends only on its own problem, so a result is the same alone or in a mix.
-struct BatchQueue {
-    int ntiles;          // tiles in the table
-    int pad;
-    unsigned counter[4]; // one work counter per kernel of a step
-};
-
+// ---- batched kernel: a persistent grid over a table of (problem, tile) pairs -----------------
+// Every problem of the batch keeps its own KArgs (state, geometry, control block).  Wavefront w of
+// the persistent grid takes tiles w, w + W, w + 2W, ... of the table (W = wavefronts in the grid):
+// no claim atomics, which serialise at one address at tens of ns each, and every wavefront's
+// loop ends after at most ceil(ntiles / W) tiles.  Neighbouring wavefront ids sit on one XCD, so
+// neighbouring tiles of a problem share an L2.  The dt of each problem comes from a wavefront-level
+// minimum and the last tile of that problem.  A cell's arithmetic depends only on its own problem,
+// so a result is the same alone or in a mix.
 __device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, int lane, int ntiles) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
@@ -530,20 +527,15 @@ __device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, 
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
 __global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict__ descs,
                                                          const int2* __restrict__ table,
-                                                         BatchQueue* q, int slot, int nslots) {
+                                                         const int* __restrict__ ntiles_p) {
     using MT = ModeTraits<MODE>;
     const int lane = threadIdx.x & 63;
-    // the previous kernel of the step has finished: its counter can be re-armed
-    if (blockIdx.x == 0 && threadIdx.x == 0) q->counter[(slot + nslots - 1) % nslots] = 0;
-    // The loop control must be wave-uniform scalars: the ticket is taken by lane 0 and moved to
-    // an SGPR with readfirstlane after the lanes reconverge.  (A ticket broadcast through a VGPR
-    // shuffle lets the compiler treat the exit as divergent and give lanes their own iterations.)
-    const int ntiles = __builtin_amdgcn_readfirstlane(q->ntiles);
-    for (int it = 0; it <= ntiles; it++) {  // at most one ticket per tile plus the final one
-        int tl = 0;
-        if (lane == 0) tl = int(atomicAdd(&q->counter[slot], 1u));
-        const int t = __builtin_amdgcn_readfirstlane(tl);
-        if (t >= ntiles) break;
+    // loop control in SGPRs: every quantity below is uniform over the wavefront
+    const int ntiles = __builtin_amdgcn_readfirstlane(*ntiles_p);
+    const int nwpb = blockDim.x >> 6;
+    const int stride = gridDim.x * nwpb;
+    const int w0 = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * nwpb + (threadIdx.x >> 6));
+    for (int t = w0; t < ntiles; t += stride) {
         const int2 e = table[t];
         const KArgs& a = descs[e.x];
         StepCtl* ctl = a.ctl;

@@ -2,8 +2,8 @@
 //
 // Every resident solve keeps its own device state (U^n in double, L(U^n) and U^2 in the
 // stage-array type), geometry and control block inside one device arena.  A step of the batch is
-// the fused-dt SSP-RK3 sequence (three launches per kernel family) over a tile work-queue that
-// spans all resident solves of that family; each solve takes its own dt, stops at its own t_end
+// the fused-dt SSP-RK3 sequence (three launches per kernel family); each launch is a persistent
+// grid over a (problem, tile) table that spans all resident solves of that family; each solve takes its own dt, stops at its own t_end
 // or step count, and its arithmetic does not depend on what else is in the batch.  Between steps
 // the engine retires finished solves (their state is copied back) and admits queued ones that
 // fit the HBM budget: continuous batching.  A background thread can drive it (serving).
@@ -121,7 +121,7 @@ struct Family {
     // device tables
     KArgs* d_desc = nullptr;
     int2* d_table = nullptr;
-    BatchQueue* d_queue = nullptr;
+    int* d_ntiles = nullptr;  // tiles in the table (read by the kernels)
     size_t cap_desc = 0, cap_table = 0;
     std::vector<long long> members;  // tickets of resident problems, in descriptor order
     bool dirty = true;
@@ -347,10 +347,8 @@ int rebuild_tables(ts_batch* b, Family& f) {
         BCHK(hipMalloc(&f.d_table, f.cap_table * sizeof(int2)));
         realloc = true;
     }
-    if (!f.d_queue) {
-        BCHK(hipMalloc(&f.d_queue, sizeof(BatchQueue)));
-        BatchQueue z{};
-        BCHK(hipMemcpy(f.d_queue, &z, sizeof(z), hipMemcpyHostToDevice));  // source on the stack
+    if (!f.d_ntiles) {
+        BCHK(hipMalloc(&f.d_ntiles, sizeof(int)));
         realloc = true;
     }
     if (realloc && f.graph) {
@@ -369,10 +367,9 @@ int rebuild_tables(ts_batch* b, Family& f) {
     }
     BCHK(hipMemcpyAsync(f.d_desc, desc.data(), desc.size() * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
     if (nt) BCHK(hipMemcpyAsync(f.d_table, table.data(), nt * sizeof(int2), hipMemcpyHostToDevice, b->stream));
-    // queue header: tile count (the counters are re-armed by the kernels themselves)
     BCHK(hipStreamSynchronize(b->stream));  // descriptor / table sources are local vectors
     int ntiles = (int)nt;
-    BCHK(hipMemcpy(&f.d_queue->ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(f.d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice));
     f.dirty = false;
     return TS_OK;
 }
@@ -384,9 +381,9 @@ namespace {
 // launches of one batched step for family f
 void issue_step(ts_batch* b, Family& f) {
     int nblk = b->ncu * f.occupancy;
-    f.ks.bf1(f.d_desc, f.d_table, f.d_queue, 0, 3, nblk, b->stream);
-    f.ks.bf2(f.d_desc + f.cap_desc, f.d_table, f.d_queue, 1, 3, nblk, b->stream);
-    f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_queue, 2, 3, nblk, b->stream);
+    f.ks.bf1(f.d_desc, f.d_table, f.d_ntiles, nblk, b->stream);
+    f.ks.bf2(f.d_desc + f.cap_desc, f.d_table, f.d_ntiles, nblk, b->stream);
+    f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_ntiles, nblk, b->stream);
 }
 
 int issue_steps(ts_batch* b, Family& f, int n) {
@@ -484,7 +481,7 @@ int ts_batch_destroy(ts_batch* b) {
         if (f.graph) hipGraphExecDestroy(f.graph);
         if (f.d_desc) hipFree(f.d_desc);
         if (f.d_table) hipFree(f.d_table);
-        if (f.d_queue) hipFree(f.d_queue);
+        if (f.d_ntiles) hipFree(f.d_ntiles);
     }
     if (b->h_ctl) hipHostFree(b->h_ctl);
     if (b->h_ptr) hipHostFree(b->h_ptr);

@@ -159,8 +159,9 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->ty = 1;
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
         c->ntiles = c->nstrips;
-        c->block_dim = dim3(64);
-        c->grid_dim = dim3(c->ntiles);
+        // four strips per block: the dt reduction costs two same-address atomics per block
+        c->block_dim = dim3(256);
+        c->grid_dim = dim3((c->ntiles + 3) / 4);
     } else if (c->nd == 2) {
         c->ty = tsr::tuned_rows_per_tile(*cfg);
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;

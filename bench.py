@@ -220,14 +220,15 @@ def traffic_from_profiles(kernel):
     path = os.path.join(ROOT, "profiles", "c2_2048_fp64_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    for line in open(path):
+    found = None
+    for line in open(path):  # the traffic table comes last; its last column is bytes per launch
         if line.startswith(f"| {kernel} |"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:
-                return float(cells[-1])
+                found = float(cells[-1])
             except ValueError:
-                return None
-    return None
+                pass
+    return found
 
 
 def main():

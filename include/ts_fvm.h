@@ -146,8 +146,9 @@ int ts_set_time(ts_ctx* ctx, double time, long long iteration);
 int ts_step(ts_ctx* ctx, int max_steps, double t_max, double next_snapshot_time,
             ts_step_info* out);
 
-/* Enable computation of the residual norms sum(V*|R|) of UpdateSolution (kernel.h:326) for
-   forward-Euler steps (off by default: they cost one extra reduction per step). */
+/* Enable the residual norms sum(V*|R|) per variable of UpdateSolution (kernel.h:326), reported
+   in ts_step_info.residual_norm for the last step taken (R of U^n: the forward-Euler residual,
+   or SSP-RK3 stage 1).  Off by default: they cost one extra residual pass per step. */
 int ts_enable_residual_norms(ts_ctx* ctx, int enable);
 
 /* Device-side log of the dt of every step (for comparing dt sequences). */
@@ -155,7 +156,8 @@ int ts_enable_dt_log(ts_ctx* ctx, long long capacity);
 int ts_get_dt_log(ts_ctx* ctx, double* out, long long capacity, long long* n_out);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch.
-   kind: 0 = dt pass, 1 = stage 1 / Euler stage, 2 = stage 2, 3 = stage 3, 4 = all. */
+   kind: 0 = dt pass, 1 = stage 1 / Euler stage, 2 = stage 2, 3 = stage 3, 4 = kinds 0-3,
+   5 = residual-norm pass. */
 int ts_enable_kernel_timing(ts_ctx* ctx, int enable);
 int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* launches);
 

@@ -260,3 +260,38 @@ def test_3d_edge_sizes(shape, periodic):
     U = tgv(cfg, amp=0.05)
     a, b, dg, dr, gi, ri = run_pair(cfg, U, 8)
     check(a, b, dg, dr, gi, ri)
+
+
+@pytest.mark.parametrize("nd,riemann,recon,stages", [
+    (1, capi.RIEMANN_ROE, capi.RECON_PWC, 1),
+    (1, capi.RIEMANN_GODUNOV, capi.RECON_LINEAR_LIM, 3),
+    (2, capi.RIEMANN_HLLC, capi.RECON_LINEAR_LIM, 3),
+    (2, capi.RIEMANN_ROE, capi.RECON_ENO2, 1),
+    (3, capi.RIEMANN_HLLC, capi.RECON_LINEAR_LIM, 3),
+])
+def test_residual_norms(nd, riemann, recon, stages):
+    """ts_step_info.residual_norm = sum(V |R|) of the last step's R(U^n) (kernel.h:326), against
+    the oracle's, with the norm pass switched on after a first chunk (graph re-capture) and the
+    state unaffected by it."""
+    shape = {1: dict(nx=300), 2: dict(nx=70, ny=50), 3: dict(nx=20, ny=14, nz=12)}[nd]
+    cfg = capi.make_config(ndims=nd, cfl=0.4, riemann=riemann, recon=recon, rk_stages=stages,
+                           bcs=[dict(type=capi.BC_NATURAL)], **shape)
+    U = util.quadrants(cfg) if nd > 1 else util.sod(cfg)
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U)
+    gpu.step(7)
+    ref.step(7)
+    gpu.enable_residual_norms(True)
+    for k in (1, 5, 300):
+        gi = gpu.step(k)
+        ri, _ = ref.step(k, log=True)
+        g, r = np.array(gi.residual_norm), np.array(ri.residual_norm)
+        assert np.all(np.abs(g - r) <= 1e-12 * np.abs(r).max()), (k, g, r)
+    gpu.enable_residual_norms(False)
+    gi = gpu.step(3)
+    ri, _ = ref.step(3, log=True)
+    assert all(x == 0.0 for x in gi.residual_norm)
+    err = util.normwise_rel(capi.interior(cfg, gpu.get_state()), capi.interior(cfg, ref.get_state()))
+    assert np.all(err <= TOL), err

@@ -194,3 +194,23 @@ def test_lax_liu_diagonal_symmetry(flux):
     S = capi.interior(cfg, o.get_state())[0]
     np.testing.assert_allclose(S[..., 0], S[..., 0].T, rtol=1e-12)
     np.testing.assert_allclose(S[..., 1], S[..., 2].T, rtol=1e-12, atol=1e-12)
+
+
+def test_residual_norm_first_sod_step():
+    """UpdateSolution's norm sum(V |R|) (kernel.h:326) on the first Sod step with Roe: only the
+    two cells beside the diaphragm have a residual, R = -(F* - F_L) and -(F_R - F*), and the mass
+    fluxes of the resting states are zero, so norm[0] = 2 h |F*_rho| with F* the Roe flux of the
+    diaphragm face (pinned separately).  A V^2 or |R|/V weighting fails this by a factor h."""
+    n = 200
+    cfg = capi.make_config(ndims=1, nx=n, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC,
+                           bcs=[dict(type=capi.BC_NATURAL)])
+    o = po.Oracle(cfg)
+    o.set_state(util.sod(cfg))
+    info, _ = o.step(1, log=True)
+    UL = util.prim2cons(1.0, 0.0, 0.0, 0.0, 1.0, 1.4)
+    UR = util.prim2cons(0.125, 0.0, 0.0, 0.0, 0.1, 1.4)
+    F, _ = po.flux(capi.RIEMANN_ROE, 1.4, UL, UR, (1.0, 0.0, 0.0))
+    h = 1.0 / n
+    assert abs(info.residual_norm[0] - 2 * h * abs(F[0])) <= 1e-15 * abs(F[0])
+    # energy: F_L and F_R of the resting states carry no energy flux either
+    assert abs(info.residual_norm[4] - 2 * h * abs(F[4])) <= 1e-14 * abs(F[4])

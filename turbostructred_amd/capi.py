@@ -30,7 +30,7 @@ BC_WALL, BC_SYMMETRY, BC_MOVING_WALL, BC_SUBSONIC_INLET, BC_SUBSONIC_OUTLET, BC_
 FP64, FP32_RESIDUAL = 0, 1
 MAX_BC = 8
 
-KIND_DT, KIND_STAGE1, KIND_STAGE2, KIND_STAGE3, KIND_ALL = 0, 1, 2, 3, 4
+KIND_DT, KIND_STAGE1, KIND_STAGE2, KIND_STAGE3, KIND_ALL, KIND_NORMS = 0, 1, 2, 3, 4, 5
 
 
 class BCConfig(C.Structure):
@@ -271,6 +271,9 @@ class Solver:
         info = StepInfo()
         self._check(self.lib.ts_step(self.h, max_steps, t_max, next_snapshot, C.byref(info)))
         return info
+
+    def enable_residual_norms(self, enable=True):
+        self._check(self.lib.ts_enable_residual_norms(self.h, int(enable)))
 
     def enable_dt_log(self, cap):
         self._check(self.lib.ts_enable_dt_log(self.h, cap))

@@ -17,6 +17,7 @@ using OccupancyFn = int (*)();  // resident 256-thread blocks per CU of the batc
 struct KernelSet {
     LaunchFn dt = nullptr, s1e = nullptr, s1r = nullptr, s2 = nullptr, s3 = nullptr;
     LaunchFn f1 = nullptr, f2 = nullptr;
+    LaunchFn rn = nullptr;  // residual norms of the stencil input (no state written)
     BatchLaunchFn bf1 = nullptr, bf2 = nullptr, bs3 = nullptr;  // fused-dt SSP-RK3, batched
     OccupancyFn bocc = nullptr;
 };

@@ -58,6 +58,7 @@ static void fill(int precision, KernelSet* ks) {
         ks->s3 = launch<RC, LM, MODE_S3, double, double, double>;
         ks->f1 = launch<RC, LM, MODE_F1, double, double, double>;
         ks->f2 = launch<RC, LM, MODE_F2, double, double, double>;
+        ks->rn = launch<RC, LM, MODE_RN, double, double, double>;
 #if TS_DIM != 3
         ks->bf1 = launch_batched<RC, LM, MODE_F1, double, double, double>;
         ks->bf2 = launch_batched<RC, LM, MODE_F2, double, double, double>;
@@ -72,6 +73,7 @@ static void fill(int precision, KernelSet* ks) {
         ks->s3 = launch<RC, LM, MODE_S3, float, float, double>;
         ks->f1 = launch<RC, LM, MODE_F1, float, double, float>;
         ks->f2 = launch<RC, LM, MODE_F2, float, double, float>;
+        ks->rn = launch<RC, LM, MODE_RN, float, double, float>;
 #if TS_DIM != 3
         ks->bf1 = launch_batched<RC, LM, MODE_F1, float, double, float>;
         ks->bf2 = launch_batched<RC, LM, MODE_F2, float, double, float>;

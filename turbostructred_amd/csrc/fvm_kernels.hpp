@@ -20,13 +20,16 @@
 //            reduction of MODE_DT from the same face eigenvalues.
 //   MODE_F2  SSP stage 2 reading Un and L(Un): its stencil values are U1 = Un + dt L(Un)
 //            formed on the fly, so U1 never goes to HBM.
+//   MODE_RN  residual norms sum(V |R|) per variable of UpdateSolution (kernel.h:326) over the
+//            interior, writing nothing else: per-block partial sums, and the last block adds
+//            them in block order (a deterministic sum).  Run only when the norms are asked for.
 #pragma once
 
 #include "fvm_device.hpp"
 
 namespace tsd {
 
-enum : int { MODE_DT = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3, MODE_F1 = 4, MODE_F2 = 5 };
+enum : int { MODE_DT = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3, MODE_F1 = 4, MODE_F2 = 5, MODE_RN = 6 };
 constexpr int STRIP = 60;   // output columns per wavefront
 constexpr int ZW = 8;         // wavefronts (z-planes) per 3-D workgroup
 constexpr int ZOUT = ZW - 2;  // output z-planes per 3-D workgroup
@@ -47,6 +50,8 @@ struct StepCtl {
     unsigned int blocks_done, pad1;
     double* dt_log;
     long long dt_log_base, dt_log_cap;  // dt of iteration `it` goes to dt_log[it - base]
+    double rnorm[5];                    // residual norms of the last step (MODE_RN)
+    unsigned int rn_done, pad2;
 };
 
 struct GridArgs {
@@ -76,6 +81,7 @@ struct KArgs {
     int ty;            // output rows per tile
     int nstrips;       // strips along x
     int ntiles;        // strips * row blocks
+    double* rn_part;   // per-block partial residual norms, 5 per block (MODE_RN)
 };
 
 // ---- wave-level helpers -------------------------------------------------------------------------
@@ -189,6 +195,38 @@ __device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block
     }
 }
 
+// Block sum of the per-lane residual norms (wave shuffle tree, then the waves in order), one
+// partial per block, and the last block to finish adds the partials in block order.
+__device__ inline void rn_reduce_finalize(StepCtl* ctl, double* part, const double (&rn)[NV], double* red,
+                                          int lane, int wave, int nwaves) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double x = rn[v];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+        if (lane == 0) red[wave * NV + v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 0; v < NV; v++) {
+            double x = red[v];
+            for (int w = 1; w < nwaves; w++) x += red[w * NV + v];
+            part[(size_t)blockIdx.x * NV + v] = x;
+        }
+        __threadfence();
+        unsigned prev = atomicAdd(&ctl->rn_done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            for (int v = 0; v < NV; v++) {
+                double x = 0.0;
+                for (unsigned b = 0; b < gridDim.x; b++) x += part[(size_t)b * NV + v];
+                ctl->rnorm[v] = x;
+            }
+            atomicExch(&ctl->rn_done, 0u);
+        }
+    }
+}
+
 template <int MODE> struct ModeTraits {
     static constexpr bool sr = MODE == MODE_DT || MODE == MODE_F1;  // needs V/SR and dt
     static constexpr bool flux = MODE != MODE_DT;                    // needs fluxes
@@ -201,7 +239,8 @@ template <int MODE> struct ModeTraits {
 // block_min receives the tile's min(V/SR) (sr modes).  `tile` < 0 marks a wavefront without work
 // that still has to take part in a block reduction.
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
-__device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min) {
+__device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min,
+                                           double* rn = nullptr) {
     using MT = ModeTraits<MODE>;
     const GridArgs& g = a.g;
     const StepCtl* ctl = a.ctl;
@@ -262,10 +301,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 
     // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
     auto finish = [&](size_t off, const R (&u)[NV], const R (&res)[NV], const double (&un)[NV],
-                      double ivol) {
+                      double ivol, double vol) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            if constexpr (MODE == MODE_F1) {
+            if constexpr (MODE == MODE_RN) {
+                if (out_lane) rn[v] += vol * __builtin_fabs(double(res[v]));
+            } else if constexpr (MODE == MODE_F1) {
                 if (out_lane) uo[v * P + off] = TOut(double(res[v]) * ivol);
             } else {
                 double q = double(u[v]) + dt * (double(res[v]) * ivol);
@@ -321,7 +362,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 R Fr = dpp_from_right(F[v]);
                 res[v] = -((Fr - F[v]) * Sx);
             }
-            finish(xs, c, res, un, ihx);
+            finish(xs, c, res, un, ihx, hx);
         }
     } else {
         // ---- march along y ----
@@ -403,7 +444,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                         R res[NV];
 #pragma unroll
                         for (int v = 0; v < NV; v++) res[v] = dfx[v] - (G[v] - gbot[v]) * Sy;
-                        finish((size_t)rf * nx + xs, Um, res, Nc, ivol);
+                        finish((size_t)rf * nx + xs, Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
                     }
 #pragma unroll
                     for (int v = 0; v < NV; v++) gbot[v] = G[v];
@@ -473,12 +514,17 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
-    if constexpr (!MT::sr) {
+    if constexpr (!MT::sr && MODE != MODE_RN) {
         if (tile >= a.ntiles) return;  // no block-level synchronisation in these modes
     }
     double block_min = __builtin_inf();
-    stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, tile < a.ntiles ? tile : -1, lane, block_min);
+    double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
     if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
+    if constexpr (MODE == MODE_RN) {
+        __shared__ double rred[4 * NV];
+        rn_reduce_finalize(ctl, a.rn_part, rn, rred, lane, wave, nwaves);
+    }
 }
 
 // ---- batched kernel: a persistent grid over a table of (problem, tile) pairs -----------------

@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     };
 
     double block_min = __builtin_inf();
+    double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int y0 = yb * a.ty;
     const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
     R Um[NV], U0[NV], Up[NV], Pn[NV];
@@ -238,7 +239,9 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         R res = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
-                        if constexpr (MODE == MODE_F1) {
+                        if constexpr (MODE == MODE_RN) {
+                            if (out_lane) rn[v] += hx * g.h[1][rf + 1] * hz * __builtin_fabs(double(res));
+                        } else if constexpr (MODE == MODE_F1) {
                             if (out_lane) uo[v * P + off] = TOut(double(res) * ivol);
                         } else {
                             double q = double(Um[v]) + dt * (double(res) * ivol);
@@ -298,6 +301,10 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         }
     }
     if constexpr (MT::sr) dt_reduce_finalize(ctl, g.cfl, block_min, red, lane, w, ZW);
+    if constexpr (MODE == MODE_RN) {
+        __shared__ double rred[ZW * NV];
+        rn_reduce_finalize(ctl, a.rn_part, rn, rred, lane, w, ZW);
+    }
 }
 
 }  // namespace tsd

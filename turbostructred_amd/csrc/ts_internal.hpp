@@ -50,8 +50,11 @@ struct ts_ctx {
 
     // kernel timing
     bool timing = false;
-    double t_ms[5] = {0, 0, 0, 0, 0};
-    long long t_n[5] = {0, 0, 0, 0, 0};
+    double t_ms[6] = {0, 0, 0, 0, 0, 0};  // per kind: dt, stages 1-3, (unused), residual norms
+    long long t_n[6] = {0, 0, 0, 0, 0, 0};
+    bool norms = false;         // residual-norm pass after the first kernel of every step
+    double* rn_part = nullptr;  // its per-block partial sums
+    double h_rnorm[5] = {0, 0, 0, 0, 0};
     std::vector<hipEvent_t> events;
 
     // hipGraph of a chunk of steps (built lazily per (chunk length, start buffer))

@@ -239,7 +239,22 @@ int ts_set_time(ts_ctx* c, double time, long long iteration) {
 
 int ts_enable_residual_norms(ts_ctx* c, int enable) {
     if (!c) return TS_ERR_INVALID;
-    if (enable) return ctx_fail(c, TS_ERR_UNSUPPORTED, "residual norms are not computed by this build");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (enable && !c->rn_part) {
+        HIPCHK(c, hipMalloc((void**)&c->rn_part, (size_t)c->grid_dim.x * 5 * sizeof(double)));
+        c->device_bytes += (size_t)c->grid_dim.x * 5 * sizeof(double);
+    }
+    if (c->norms != (enable != 0)) {
+        // the captured step graphs hold the launch sequence: re-capture with or without the pass
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < 2; i++) {
+            if (c->graph_exec[i]) HIPCHK(c, hipGraphExecDestroy(c->graph_exec[i]));
+            c->graph_exec[i] = nullptr;
+            c->graph_steps[i] = 0;
+        }
+    }
+    c->norms = enable != 0;
+    for (int v = 0; v < 5; v++) c->h_rnorm[v] = 0.0;
     return TS_OK;
 }
 
@@ -268,12 +283,12 @@ int ts_get_dt_log(ts_ctx* c, double* out, long long capacity, long long* n_out) 
 int ts_enable_kernel_timing(ts_ctx* c, int enable) {
     if (!c) return TS_ERR_INVALID;
     c->timing = enable != 0;
-    for (int k = 0; k < 5; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    for (int k = 0; k < 6; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
     return TS_OK;
 }
 
 int ts_get_kernel_timing(ts_ctx* c, int kind, double* total_ms, long long* launches) {
-    if (!c || kind < 0 || kind > 4 || !total_ms || !launches) return TS_ERR_INVALID;
+    if (!c || kind < 0 || kind > 5 || !total_ms || !launches) return TS_ERR_INVALID;
     if (kind == 4) {
         double s = 0;
         long long n = 0;
@@ -326,7 +341,7 @@ int ts_step(ts_ctx* c, int max_steps, double t_max, double next_snapshot_time, t
         out->time = c->h_time;
         out->dt = c->h_dt;
         out->iteration = c->h_iter;
-        for (int v = 0; v < 5; v++) out->residual_norm[v] = 0.0;
+        for (int v = 0; v < 5; v++) out->residual_norm[v] = c->h_rnorm[v];
         out->steps_done = c->h_steps_done;
     }
     return rc;

@@ -191,6 +191,7 @@ void destroy_device_ctx(ts_ctx* c) {
     for (double* p : c->dbufs) hipFree(p);
     c->dbufs.clear();
     if (c->dt_log) hipFree(c->dt_log);
+    if (c->rn_part) hipFree(c->rn_part);
     if (c->hctl) hipHostFree(c->hctl);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -257,7 +258,7 @@ struct Launch {
 };
 
 // the launches of step s of a chunk, given the buffer holding U^n at the chunk start
-void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[4], int& n) {
+void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[5], int& n) {
     KArgs a{};
     a.g = c->grid;
     a.plane = c->ncell;
@@ -266,18 +267,30 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[4], int& n
     a.nstrips = c->nstrips;
     a.ntiles = c->ntiles;
     a.un = (const double*)c->dU[0];
+    a.rn_part = c->rn_part;
     n = 0;
+    // residual norms of U^n, after the kernel that decides whether the step runs
+    auto norms = [&](const void* un_buf) {
+        if (!c->norms) return;
+        KArgs r = a;
+        r.uin = un_buf;
+        r.lin = nullptr;
+        r.uout = nullptr;
+        L[n++] = {c->ks.rn, r, 5};
+    };
     if (c->cfg.rk_stages == 1) {
         int in = (start_buf + s) & 1, out = in ^ 1;
         a.uin = c->dU[in];
         a.uout = nullptr;
         L[n++] = {c->ks.dt, a, 0};
+        norms(c->dU[in]);
         a.uout = c->dU[out];
         L[n++] = {c->ks.s1e, a, 1};
     } else if (c->fused_dt) {
         a.uin = c->dU[0];
         a.uout = c->dU[1];
         L[n++] = {c->ks.f1, a, 1};  // L(U^n) and dt
+        norms(c->dU[0]);
         a.lin = c->dU[1];
         a.uout = c->dU[2];
         L[n++] = {c->ks.f2, a, 2};  // U2 from U^n + dt L(U^n) on the fly
@@ -288,6 +301,7 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[4], int& n
     } else {
         a.uin = c->dU[0];
         L[n++] = {c->ks.dt, a, 0};
+        norms(c->dU[0]);
         a.uout = c->dU[1];
         L[n++] = {c->ks.s1r, a, 1};
         a.uin = c->dU[1];
@@ -309,7 +323,7 @@ int issue_direct(ts_ctx* c, int start_buf, int first, int count) {
     }
     size_t ev = 0;
     for (int s = first; s < first + count; s++) {
-        Launch L[4];
+        Launch L[5];
         int n;
         step_launches(c, start_buf, s, L, n);
         for (int i = 0; i < n; i++) {
@@ -347,7 +361,7 @@ int issue_graph(ts_ctx* c, int start_buf, int count) {
         hipGraph_t graph;
         CHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < count; s++) {
-            Launch L[4];
+            Launch L[5];
             int n;
             step_launches(c, start_buf, s, L, n);
             for (int i = 0; i < n; i++) L[i].fn(L[i].args, c->grid_dim, c->block_dim, c->stream);
@@ -411,6 +425,7 @@ int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap) {
     c->h_dt = h->dt;
     c->h_iter = h->iter;
     c->h_steps_done = h->steps_done;
+    for (int v = 0; v < 5; v++) c->h_rnorm[v] = c->norms ? h->rnorm[v] : 0.0;
     if (c->cfg.rk_stages == 1) c->cur = (start_buf + h->steps_done) & 1;
     if (h->error) {
         c->err = "dt is not finite and positive (non-physical state)";

@@ -1,0 +1,90 @@
+"""The C++ host program path: a shock-tube driver written like the reference's own
+(src/Tests/tests.h:13-80, src/Tests/1DTests/ToroTests.h) over the HipExplicitRungeKuttaFVM class,
+run as a separate process, against the oracle driven with the same grid, stop rule and snapshot
+clamps (kernel.h:1701-1850, ERK:1433).  Bar: state and time to 1e-12 relative, same step count,
+same number of snapshot stops."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from turbostructred_amd import capi
+from oracle import pyoracle
+import util
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "turbostructred_amd", "host", "shock_tube_driver")
+TORO = json.load(open(os.path.join(ROOT, "tests", "golden", "toro_star_table.json")))["tests"]
+
+
+def run_driver(tmp_path, left, right, x0, t_end, nx, recon, riemann, rk, compress, save_time, norms):
+    if not os.path.exists(DRIVER):
+        raise RuntimeError(f"{DRIVER} is missing: build it (make -C turbostructred_amd/csrc)")
+    out = str(tmp_path / "out.bin")
+    args = [DRIVER, *map(repr, left), *map(repr, right), repr(x0), repr(t_end), str(nx), str(recon),
+            str(riemann), str(rk), str(compress), repr(save_time), str(norms), out]
+    print("driver:", " ".join(args[1:-1]), flush=True)
+    subprocess.run(args, check=True, timeout=120)
+    raw = open(out, "rb").read()
+    hdr = np.frombuffer(raw[:16], dtype=np.int32)
+    nxa, snaps, iters = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    f = np.frombuffer(raw[16:], dtype=np.float64)
+    st = f[:7]
+    X, H = f[7:7 + nxa], f[7 + nxa:7 + 2 * nxa]
+    U = f[7 + 2 * nxa:].reshape(1, 1, nxa, 5)
+    return dict(iters=iters, snaps=snaps, time=st[0], dt=st[1], norms=st[2:7], X=X.copy(), H=H.copy(), U=U)
+
+
+def run_oracle(cfg, U, t_end, save_time):
+    """Kernel::Run with SaveSolutionTime (kernel.h:1701-1850) on the oracle."""
+    o = pyoracle.Oracle(cfg)
+    o.set_state(U)
+    nxt, snaps, it = save_time, 0, 0
+    while True:
+        info, _ = o.step(1 << 20, t_end, nxt if save_time > 0 else 0.0, log=True)
+        it += info.steps_done
+        if save_time > 0 and info.time == nxt:
+            snaps += 1
+            nxt += save_time
+        if info.steps_done == 0 or info.time >= t_end:
+            break
+    return o, info, snaps, it
+
+
+CASES = [  # test, nx, recon, riemann, rk, compress, save_time, norms
+    ("1", 100, capi.RECON_LINEAR_LIM, capi.RIEMANN_ROE, 1, 1, 0.05, 1),   # tests.h:13-80 grid, snapshots
+    ("1", 400, capi.RECON_PWC, capi.RIEMANN_GODUNOV, 1, 0, 0.0, 0),
+    ("2", 300, capi.RECON_PWC, capi.RIEMANN_GODUNOV, 3, 0, 0.05, 1),     # near-vacuum: first order
+    ("3", 300, capi.RECON_ENO2, capi.RIEMANN_HLLC, 3, 0, 0.0, 0),
+    ("4", 200, capi.RECON_PWC, capi.RIEMANN_ROE, 1, 1, 0.01, 0),
+    ("5", 250, capi.RECON_LINEAR_LIM, capi.RIEMANN_HLLC, 3, 0, 0.0, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"toro{c[0]}-n{c[1]}-r{c[2]}-f{c[3]}-rk{c[4]}-c{c[5]}" for c in CASES])
+def test_driver_matches_oracle(tmp_path, case):
+    t, nx, recon, riemann, rk, compress, save_time, norms = case
+    T = TORO[t]
+    d = run_driver(tmp_path, T["left"], T["right"], T["x0"], T["t"], nx, recon, riemann, rk, compress,
+                   save_time, norms)
+    cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=riemann, recon=recon, rk_stages=rk,
+                           bcs=[dict(type=capi.BC_NATURAL)], coord=[d["X"], None, None], h=[d["H"], None, None])
+    xc = d["X"][1:-1]
+    left = xc < T["x0"]
+    rho = np.where(left, T["left"][0], T["right"][0])
+    u = np.where(left, T["left"][1], T["right"][1])
+    p = np.where(left, T["left"][2], T["right"][2])
+    U = np.zeros((1, 1, nx + 2, 5))
+    U[0, 0, 1:-1] = util.prim2cons(rho, u, 0 * u, 0 * u, p, 1.4)
+    o, info, snaps, it = run_oracle(cfg, U, T["t"], save_time)
+    assert d["iters"] == it and d["snaps"] == snaps, (d["iters"], it, d["snaps"], snaps)
+    assert abs(d["time"] - info.time) <= 1e-12 * info.time
+    assert d["time"] >= T["t"]
+    err = util.normwise_rel(capi.interior(cfg, d["U"]), capi.interior(cfg, o.get_state()))
+    assert np.all(err <= 1e-12), err
+    if norms:
+        r = np.array(info.residual_norm)
+        assert np.all(np.abs(d["norms"] - r) <= 1e-12 * np.abs(r).max())

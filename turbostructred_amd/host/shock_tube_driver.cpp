@@ -1,0 +1,98 @@
+// shock_tube_driver.cpp -- a 1-D shock-tube run written the way the reference's own drivers are
+// (src/Tests/tests.h:13-80 RunSODTestRoe1D, src/Tests/1DTests/ToroTests.h), against the
+// HipExplicitRungeKuttaFVM host class: KernelConfiguration, CreateKernel, Init,
+// SetInitialConditions, Run.
+//
+// usage: shock_tube_driver rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress
+//                          save_time residual_norms out.bin
+//   recon / riemann: the reference's enum values; rk: 1 or 3; compress: 1 = the two-block
+//   geometric grid of RunSODTestRoe1D (q = 1/1.05 then 1.05, node at 0.5 L), 0 = uniform;
+//   save_time > 0 clamps dt onto its multiples (SaveSolutionTime).
+// out.bin: int32 nXAll, int32 snapshots, int32 iteration, pad; double time, dt, residual[5];
+//          CoordinateX[nXAll], hx[nXAll], values[nXAll*5].
+#include <cstdio>
+#include <cstdlib>
+#include <exception>
+#include <vector>
+
+#include "ts_kernel.hpp"
+
+using namespace tsh;
+
+int main(int argc, char** argv) {
+    if (argc != 17) {
+        std::fprintf(stderr, "usage: %s rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress save_time norms out.bin\n",
+                     argv[0]);
+        return 2;
+    }
+    const double roL = std::atof(argv[1]), uL = std::atof(argv[2]), PL = std::atof(argv[3]);
+    const double roR = std::atof(argv[4]), uR = std::atof(argv[5]), PR = std::atof(argv[6]);
+    const double x0 = std::atof(argv[7]), t_end = std::atof(argv[8]);
+    const int nx = std::atoi(argv[9]), recon = std::atoi(argv[10]), riemann = std::atoi(argv[11]);
+    const int rk = std::atoi(argv[12]), compress = std::atoi(argv[13]);
+    const double save_time = std::atof(argv[14]);
+    const int norms = std::atoi(argv[15]);
+    const char* out = argv[16];
+    try {
+        KernelConfiguration conf;
+        conf.nDims = 1;
+        conf.LX = 1.0;
+        conf.nX = nx;
+        conf.isPeriodicX = false;
+        conf.DummyLayerSize = 1;
+        if (compress) {
+            BlockNode nleft, ncenter;
+            nleft.N_cells = conf.nX / 2;
+            nleft.q_com = 1.0 / 1.05;
+            ncenter.pos = 0.5 * conf.LX;
+            ncenter.N_cells = conf.nX - nleft.N_cells;
+            ncenter.q_com = 1.0 / nleft.q_com;
+            conf.CompressionX[0] = nleft;
+            conf.CompressionX.push_back(ncenter);
+        }
+        conf.MyConditions[1] = BoundaryConditionConfiguration(BoundaryConditionType::Natural);
+        conf.xLeftBoundary.SetMarker(1);
+        conf.xRightBoundary.SetMarker(1);
+        conf.Gamma = 1.4;
+        conf.methodConfiguration.CFL = 0.4;
+        conf.methodConfiguration.RungeKuttaOrder = rk;
+        conf.methodConfiguration.Eps = 0.0;
+        conf.methodConfiguration.ReconstructionType = static_cast<Reconstruction>(recon);
+        conf.methodConfiguration.GeneralLimitter = LimiterType::BarsJespersen;
+        conf.methodConfiguration.RiemannProblemSolver = static_cast<RPSolver>(riemann);
+        conf.MaxTime = t_end;
+        conf.MaxIteration = 1000000;
+        conf.SaveSolutionTime = save_time;
+
+        auto kernel = CreateKernel(conf);
+        kernel->Init(conf);
+        kernel->ComputeResidualNorms = norms != 0;
+        const double g1 = conf.Gamma - 1.0;
+        kernel->SetInitialConditions([&](const Vector& r) {
+            const bool left = r.x < x0;  // tests.h:85
+            const double ro = left ? roL : roR, u = left ? uL : uR, p = left ? PL : PR;
+            return std::vector<double>{ro, ro * u, 0.0, 0.0, p / g1 + 0.5 * ro * u * u};
+        });
+        int snapshots = 0;
+        kernel->OnSolutionSnapshot = [&](HipExplicitRungeKuttaFVM&) { snapshots++; };
+        kernel->Run();
+        kernel->SyncValues();
+
+        FILE* f = std::fopen(out, "wb");
+        if (!f) throw std::runtime_error("cannot open output file");
+        int hdr[4] = {kernel->nXAll, snapshots, kernel->stepInfo.Iteration, 0};
+        double st[7] = {kernel->stepInfo.Time, kernel->stepInfo.TimeStep};
+        for (int v = 0; v < 5; v++) st[2 + v] = kernel->stepInfo.Residual[v];
+        std::fwrite(hdr, sizeof(int), 4, f);
+        std::fwrite(st, sizeof(double), 7, f);
+        std::fwrite(kernel->CoordinateX.data(), sizeof(double), kernel->nXAll, f);
+        std::fwrite(kernel->hx.data(), sizeof(double), kernel->nXAll, f);
+        std::fwrite(&kernel->values[0], sizeof(double), kernel->values.size(), f);
+        std::fclose(f);
+        std::printf("steps %d time %.17g snapshots %d\n", kernel->stepInfo.Iteration, kernel->stepInfo.Time, snapshots);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,407 @@
+// ts_kernel.hpp -- C++ host class over libts_fvm.so with the shape of TurboStructured's
+// Kernel / ExplicitRungeKuttaFVM<R> (src/kernel/kernel.h:42-1850, src/Methods/ExplicitRungeKuttaFVM.h).
+//
+// A host program written against the reference builds a KernelConfiguration, calls
+// CreateKernel, Init, SetInitialConditions and Run, and reads `values` / `stepInfo`.  The same
+// program runs on an MI355X by switching its includes to this header: configuration fields,
+// enum members, the step information and the termination rule of Run keep the reference's
+// names and meaning.  The step itself (ghost fill, reconstruction, Riemann fluxes, time step,
+// update) runs in libts_fvm.so; the state lives in HBM and `values` is a host mirror refreshed
+// on demand (SyncValues) and before every snapshot hook.
+//
+// Errors: the reference reports failures with exceptions and asserts; here every failing
+// library call throws std::runtime_error carrying the library's message.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <valarray>
+#include <vector>
+
+#include "../../include/ts_fvm.h"
+
+namespace tsh {
+
+// utility/Vector.h
+struct Vector {
+    double x = 0, y = 0, z = 0;
+    Vector() = default;
+    Vector(double a, double b, double c) : x(a), y(b), z(c) {}
+};
+
+// src/Methods/MethodConfiguration.h:5-23; HLLCSolver is an extension (no reference member)
+enum class Reconstruction { PiecewiseConstant, Linear2PointsStencil, ENO2PointsStencil, Linear2psLim };
+enum class RPSolver { GodunovSolver, RoePikeSolver, NoSolver, HLLCSolver };
+enum class LimiterType { BarsJespersen, Venkatakrishnan };
+
+// src/Methods/MethodConfiguration.h:26-37 (+ arithmetic of the residual, an extension)
+struct MethodConfiguration {
+    RPSolver RiemannProblemSolver{RPSolver::NoSolver};
+    Reconstruction ReconstructionType{Reconstruction::PiecewiseConstant};
+    LimiterType GeneralLimitter{LimiterType::BarsJespersen};
+    int RungeKuttaOrder{0};  // 0 or 1 = forward Euler (the reference's step), 3 = SSP-RK3
+    double CFL{0.25};
+    double Eps{0.0};
+    int Precision{TS_FP64};  // TS_FP64 or TS_FP32_RESIDUAL
+};
+
+// src/BoundaryConditions/BoundaryConditionConfiguration.h:12-74
+enum class BoundaryConditionType { Wall, Symmetry, MovingWall, SubsonicInlet, SubsonicOutlet, Natural };
+
+struct BoundaryConditionConfiguration {
+    BoundaryConditionType BCType{BoundaryConditionType::Natural};
+    Vector Velocity;
+    double Pstatic{0};
+    double Density{0};
+    BoundaryConditionConfiguration() = default;
+    explicit BoundaryConditionConfiguration(BoundaryConditionType t) : BCType(t) {}
+};
+
+class DomainBCinfo {
+   public:
+    bool isComplex{false};
+    std::function<int(const Vector r)> getMarker;
+    void SetMarker(int m) { bcMarker = m; }
+    int GetMarker() const { return bcMarker; }
+
+   private:
+    int bcMarker{0};
+};
+
+// src/kernel/KernelConfiguration.h:10-18
+struct BlockNode {
+    double pos{0.0};
+    double q_com{1.0};
+    int N_cells{-1};
+    BlockNode() = default;
+    explicit BlockNode(double p) : pos(p) {}
+};
+
+// src/kernel/KernelConfiguration.h:21-93: the fields of the inviscid explicit path
+struct KernelConfiguration {
+    int nDims{1};
+    int nX{1}, nY{1}, nZ{1};
+    bool isPeriodicX{true}, isPeriodicY{true}, isPeriodicZ{true};
+    double LX{1.0}, LY{1.0}, LZ{1.0};
+    std::vector<BlockNode> CompressionX{std::vector<BlockNode>(1)};
+    std::vector<BlockNode> CompressionY{std::vector<BlockNode>(1)};
+    std::vector<BlockNode> CompressionZ{std::vector<BlockNode>(1)};
+    double Gamma{1.4};
+    bool IsViscousFlow{false};
+    int DummyLayerSize{1};
+    MethodConfiguration methodConfiguration;
+    int MaxIteration{1000};
+    double MaxTime{1.0};
+    double SaveSolutionTime{0};
+    double SaveSliceTime{0};
+    int SaveSolutionIters{0};
+    std::map<int, BoundaryConditionConfiguration> MyConditions;
+    DomainBCinfo xLeftBoundary, xRightBoundary, yLeftBoundary, yRightBoundary, zLeftBoundary, zRightBoundary;
+    int Device{0};  // HIP device ordinal (extension)
+};
+
+// src/kernel/kernel.h:31-39
+struct StepInfo {
+    double Time{0};
+    double TimeStep{0};
+    int Iteration{0};
+    std::vector<double> Residual{std::vector<double>(5, 0.0)};
+    double NextSolutionSnapshotTime{0};
+    double NextSliceSnapshotTime{0};
+};
+
+// Cell-centre coordinates and sizes along one axis, ghost layers included, from the block
+// compression description: the construction of Grid::InitLocal (src/kernel/grid.h:164-218),
+// positions advanced as running sums from the left ghost layer.
+inline void BuildAxis(std::vector<BlockNode> blocks, int n, double L, int ghost, std::vector<double>& X,
+                      std::vector<double>& H) {
+    X.assign(n + 2 * ghost, 0.0);
+    H.assign(n + 2 * ghost, 0.0);
+    blocks.push_back(BlockNode(L));
+    double stp;
+    if (blocks[0].N_cells < 1) {
+        stp = L / n;
+        blocks[0].N_cells = n;
+    } else {
+        const double len = blocks[1].pos - blocks[0].pos;
+        const int nc = blocks[0].N_cells;
+        stp = blocks[0].q_com == 1.0 ? len / nc : len * (1.0 - blocks[0].q_com) / (1.0 - std::pow(blocks[0].q_com, nc));
+    }
+    double xc = -(ghost * stp) + 0.5 * stp;
+    int c = 0;
+    for (; c < ghost; c++) {
+        X[c] = xc;
+        H[c] = stp;
+        xc += stp;
+    }
+    for (size_t b = 1; b < blocks.size(); b++) {
+        const BlockNode& nd = blocks[b - 1];
+        for (int i = 0; i < nd.N_cells; i++) {
+            X[c + i] = xc;
+            H[c + i] = stp;
+            xc += 0.5 * stp * (1.0 + nd.q_com);
+            stp *= nd.q_com;
+        }
+        c += nd.N_cells;
+        if (blocks[b].N_cells > 0) {
+            const double len = blocks[b + 1].pos - blocks[b].pos;
+            const int nc = blocks[b].N_cells;
+            const double q = blocks[b].q_com;
+            stp = q == 1.0 ? len / nc : len * (1.0 - q) / (1.0 - std::pow(q, nc));
+            xc = blocks[b].pos + 0.5 * stp;
+        }
+    }
+    if (c != n + ghost) throw std::runtime_error("grid blocks do not add up to the cell count");
+    for (int i = 0; i < ghost; i++) {
+        X[c + i] = xc;
+        H[c + i] = stp;
+        xc += stp;
+    }
+}
+
+// The explicit finite-volume method on one GPU: Kernel + ExplicitRungeKuttaFVM<R> with the
+// reconstruction chosen at run time (the reference picks it as a template argument in
+// CreateKernel, ExplicitRungeKuttaFVM.h:1495-1515).
+class HipExplicitRungeKuttaFVM {
+   public:
+    int nDims{1};
+    int nVariables{5};
+    int nX{1}, nY{1}, nZ{1};
+    int nXAll{1}, nYAll{1}, nZAll{1};
+    std::vector<double> CoordinateX, CoordinateY, CoordinateZ, hx, hy, hz;
+    StepInfo stepInfo;
+    std::valarray<double> values;  // host mirror of the state (reference AoS layout)
+    int MaxIteration{1000};
+    double MaxTime{1.0};
+    double SaveSolutionTime{0};
+    double SaveSliceTime{0};
+    bool ComputeResidualNorms{false};
+    // called at every solution / slice snapshot time with `values` synchronised
+    std::function<void(HipExplicitRungeKuttaFVM&)> OnSolutionSnapshot, OnSliceSnapshot;
+
+    HipExplicitRungeKuttaFVM() = default;
+    HipExplicitRungeKuttaFVM(const HipExplicitRungeKuttaFVM&) = delete;
+    HipExplicitRungeKuttaFVM& operator=(const HipExplicitRungeKuttaFVM&) = delete;
+    ~HipExplicitRungeKuttaFVM() {
+        if (ctx_) ts_destroy(ctx_);
+    }
+
+    // Kernel::Init (kernel.h:178-281) + InitializeMethod: grid, boundary conditions, method,
+    // run parameters; creates the device context.
+    void Init(KernelConfiguration& config) {
+        if (config.IsViscousFlow) throw std::runtime_error("viscous flow is not part of this library");
+        nDims = config.nDims;
+        nX = config.nX;
+        nY = nDims > 1 ? config.nY : 1;
+        nZ = nDims > 2 ? config.nZ : 1;
+        const int g = config.DummyLayerSize;
+        nXAll = nX + 2 * g;
+        nYAll = nDims > 1 ? nY + 2 * g : 1;
+        nZAll = nDims > 2 ? nZ + 2 * g : 1;
+        BuildAxis(config.CompressionX, nX, config.LX, g, CoordinateX, hx);
+        if (nDims > 1) BuildAxis(config.CompressionY, nY, config.LY, g, CoordinateY, hy);
+        if (nDims > 2) BuildAxis(config.CompressionZ, nZ, config.LZ, g, CoordinateZ, hz);
+
+        ts_config c{};
+        c.ndims = nDims;
+        c.nx = nX;
+        c.ny = nY;
+        c.nz = nZ;
+        c.ghost = g;
+        c.lx = config.LX;
+        c.ly = config.LY;
+        c.lz = config.LZ;
+        c.periodic[0] = config.isPeriodicX;
+        c.periodic[1] = nDims > 1 && config.isPeriodicY;
+        c.periodic[2] = nDims > 2 && config.isPeriodicZ;
+        c.coord[0] = CoordinateX.data();
+        c.h[0] = hx.data();
+        if (nDims > 1) c.coord[1] = CoordinateY.data(), c.h[1] = hy.data();
+        if (nDims > 2) c.coord[2] = CoordinateZ.data(), c.h[2] = hz.data();
+        const MethodConfiguration& m = config.methodConfiguration;
+        c.gamma = config.Gamma;
+        c.cfl = m.CFL;
+        c.eps = m.Eps;
+        c.riemann = static_cast<int>(m.RiemannProblemSolver);
+        c.recon = static_cast<int>(m.ReconstructionType);
+        c.limiter = static_cast<int>(m.GeneralLimitter);
+        c.rk_stages = m.RungeKuttaOrder == 3 ? 3 : 1;
+        c.precision = m.Precision;
+        c.device = config.Device;
+
+        // boundary conditions: the reference's markers (any int) become table slots 0..7
+        std::map<int, int> slot;
+        for (auto& kv : config.MyConditions) {
+            if (c.n_bc == TS_MAX_BC) throw std::runtime_error("more boundary conditions than the library's table holds");
+            ts_bc_config& b = c.bc[c.n_bc];
+            b.type = static_cast<int>(kv.second.BCType);
+            b.velocity[0] = kv.second.Velocity.x;
+            b.velocity[1] = kv.second.Velocity.y;
+            b.velocity[2] = kv.second.Velocity.z;
+            b.pstatic = kv.second.Pstatic;
+            b.density = kv.second.Density;
+            slot[kv.first] = c.n_bc++;
+        }
+        auto lookup = [&](int marker) {
+            auto it = slot.find(marker);
+            if (it == slot.end()) throw std::runtime_error("boundary marker without a condition: " + std::to_string(marker));
+            return it->second;
+        };
+        const DomainBCinfo* sides[6] = {&config.xLeftBoundary, &config.xRightBoundary, &config.yLeftBoundary,
+                                        &config.yRightBoundary, &config.zLeftBoundary, &config.zRightBoundary};
+        const bool periodic[3] = {c.periodic[0] != 0, c.periodic[1] != 0, c.periodic[2] != 0};
+        for (int s = 0; s < 6; s++) {
+            const int axis = s / 2;
+            if (axis >= nDims || periodic[axis]) continue;
+            const DomainBCinfo& d = *sides[s];
+            if (!d.isComplex) {
+                c.side_marker[s] = lookup(d.GetMarker());
+                continue;
+            }
+            // composite side: the marker function at every boundary face centre (kernel.h:742,759)
+            std::vector<uint8_t>& fm = face_markers_[s];
+            const int na = axis == 0 ? (nDims > 1 ? nY : 1) : nX;
+            const int nb = axis == 2 ? (nDims > 1 ? nY : 1) : (nDims > 2 ? nZ : 1);
+            fm.assign((size_t)na * nb, 0);
+            const double xb = s == 0 ? CoordinateX[g] - 0.5 * hx[g] : CoordinateX[g + nX - 1] + 0.5 * hx[g + nX - 1];
+            for (int b2 = 0; b2 < nb; b2++)
+                for (int a2 = 0; a2 < na; a2++) {
+                    Vector r;
+                    if (axis == 0) {
+                        r = Vector(xb, nDims > 1 ? CoordinateY[g + a2] : 0.0, nDims > 2 ? CoordinateZ[g + b2] : 0.0);
+                    } else if (axis == 1) {
+                        const double yb = s == 2 ? CoordinateY[g] - 0.5 * hy[g] : CoordinateY[g + nY - 1] + 0.5 * hy[g + nY - 1];
+                        r = Vector(CoordinateX[g + a2], yb, nDims > 2 ? CoordinateZ[g + b2] : 0.0);
+                    } else {
+                        const double zb = s == 4 ? CoordinateZ[g] - 0.5 * hz[g] : CoordinateZ[g + nZ - 1] + 0.5 * hz[g + nZ - 1];
+                        r = Vector(CoordinateX[g + a2], CoordinateY[g + b2], zb);
+                    }
+                    fm[(size_t)b2 * na + a2] = (uint8_t)lookup(d.getMarker(r));
+                }
+            c.face_marker[s] = fm.data();
+        }
+
+        MaxIteration = config.MaxIteration;
+        MaxTime = config.MaxTime;
+        SaveSolutionTime = config.SaveSolutionTime;
+        SaveSliceTime = config.SaveSliceTime;
+        stepInfo = StepInfo();
+        stepInfo.NextSolutionSnapshotTime = SaveSolutionTime;  // kernel.h:261-262
+        stepInfo.NextSliceSnapshotTime = SaveSliceTime;
+
+        if (ctx_) ts_destroy(ctx_), ctx_ = nullptr;
+        int rc = ts_create(&c, &ctx_);
+        if (rc != TS_OK) {
+            std::string msg = ctx_ ? ts_last_error(ctx_) : "invalid configuration";
+            if (ctx_) ts_destroy(ctx_), ctx_ = nullptr;
+            throw std::runtime_error("ts_create failed (" + std::to_string(rc) + "): " + msg);
+        }
+        values.resize((size_t)nXAll * nYAll * nZAll * nVariables, 0.0);
+        check(ts_set_time(ctx_, 0.0, 0));
+    }
+
+    // Kernel::SetInitialConditions with cell-centre point values (kernel.h:125-146)
+    void SetInitialConditions(const std::function<std::vector<double>(const Vector& r)>& initF) {
+        const int g = 1;
+        for (int k = 0; k < nZ; k++)
+            for (int j = 0; j < nY; j++)
+                for (int i = 0; i < nX; i++) {
+                    const int I = i + g, J = nDims > 1 ? j + g : 0, K = nDims > 2 ? k + g : 0;
+                    Vector r(CoordinateX[I], nDims > 1 ? CoordinateY[J] : 0.0, nDims > 2 ? CoordinateZ[K] : 0.0);
+                    std::vector<double> U = initF(r);
+                    if ((int)U.size() < nVariables) throw std::runtime_error("initial condition returned too few values");
+                    const size_t s = (((size_t)K * nYAll + J) * nXAll + I) * nVariables;
+                    for (int v = 0; v < nVariables; v++) values[s + v] = U[v];
+                }
+        UploadValues();
+    }
+
+    // push the host mirror to the device (after editing `values` by hand)
+    void UploadValues() { check(ts_set_state(ctx_, &values[0], values.size())); }
+    // refresh the host mirror from the device
+    void SyncValues() { check(ts_get_state(ctx_, &values[0], values.size())); }
+
+    // ExplicitRungeKuttaFVM::IterationStep (ERK:1440-1491): one step
+    void IterationStep() { Advance(1); }
+
+    // Kernel::Run (kernel.h:1701-1850): steps until Iteration == MaxIteration or
+    // Time >= MaxTime, with dt clamped onto the solution / slice snapshot times (ERK:1433-1434)
+    // and the snapshot hooks called there.  Steps between snapshots run on the device without
+    // a host round trip.
+    void Run() {
+        EnableNorms();
+        while (true) {
+            // the reference steps first and tests afterwards, so a run always takes a step, and
+            // one started past MaxIteration only ends on time
+            const int left = stepInfo.Iteration < MaxIteration ? MaxIteration - stepInfo.Iteration : 1 << 30;
+            ts_step_info info{};
+            check(ts_step(ctx_, left, MaxTime, NextSnapshot(), &info));
+            Absorb(info);
+            if (info.steps_done == 0) break;
+            Snapshots();
+            if (stepInfo.Iteration == MaxIteration || stepInfo.Time >= MaxTime) break;
+        }
+    }
+
+    ts_ctx* Context() const { return ctx_; }
+
+   private:
+    ts_ctx* ctx_ = nullptr;
+    bool norms_on_ = false;
+    std::vector<uint8_t> face_markers_[6];
+
+    void check(int rc) {
+        if (rc != TS_OK) throw std::runtime_error("libts_fvm: " + std::to_string(rc) + ": " + ts_last_error(ctx_));
+    }
+    void EnableNorms() {
+        if (ComputeResidualNorms != norms_on_) {
+            check(ts_enable_residual_norms(ctx_, ComputeResidualNorms ? 1 : 0));
+            norms_on_ = ComputeResidualNorms;
+        }
+    }
+    double NextSnapshot() const {
+        double t = 0.0;
+        if (SaveSolutionTime > 0) t = stepInfo.NextSolutionSnapshotTime;
+        if (SaveSliceTime > 0 && (t == 0.0 || stepInfo.NextSliceSnapshotTime < t)) t = stepInfo.NextSliceSnapshotTime;
+        return t;
+    }
+    void Absorb(const ts_step_info& info) {
+        stepInfo.Time = info.time;
+        stepInfo.TimeStep = info.dt;
+        stepInfo.Iteration = (int)info.iteration;
+        for (int v = 0; v < nVariables; v++) stepInfo.Residual[v] = info.residual_norm[v];
+    }
+    void Advance(int n) {
+        EnableNorms();
+        ts_step_info info{};
+        check(ts_step(ctx_, n, 1e300, NextSnapshot(), &info));
+        Absorb(info);
+        Snapshots();
+    }
+    void Snapshots() {
+        const bool sol = SaveSolutionTime > 0 && stepInfo.NextSolutionSnapshotTime == stepInfo.Time;
+        const bool slc = SaveSliceTime > 0 && stepInfo.NextSliceSnapshotTime == stepInfo.Time;
+        if (!sol && !slc) return;
+        SyncValues();
+        if (sol) {
+            if (OnSolutionSnapshot) OnSolutionSnapshot(*this);
+            stepInfo.NextSolutionSnapshotTime += SaveSolutionTime;
+        }
+        if (slc) {
+            if (OnSliceSnapshot) OnSliceSnapshot(*this);
+            stepInfo.NextSliceSnapshotTime += SaveSliceTime;
+        }
+    }
+};
+
+// CreateKernel (ExplicitRungeKuttaFVM.h:1495-1515)
+inline std::unique_ptr<HipExplicitRungeKuttaFVM> CreateKernel(KernelConfiguration&) {
+    return std::unique_ptr<HipExplicitRungeKuttaFVM>(new HipExplicitRungeKuttaFVM());
+}
+
+}  // namespace tsh

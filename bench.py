@@ -34,7 +34,9 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
                            precision=precision, bcs=[dict(type=capi.BC_NATURAL)])
     U = lax_liu_state(cfg)
     s = capi.Solver(cfg)
+    t0 = time.perf_counter()
     s.set_state(U)
+    t_set = time.perf_counter() - t0
     if warmup:
         s.step(warmup)
     try:
@@ -65,8 +67,63 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
         kern[name] = dict(avg_us=avg * 1e6, launches=cnt, algorithmic_bytes=b, achieved_gbs=b / avg / 1e9,
                           frac=b / avg / HBM_PEAK)
     s.enable_kernel_timing(False)
+    t0 = time.perf_counter()
     state = s.get_state()
-    return cfg, elapsed, info, kern, state
+    t_get = time.perf_counter() - t0
+    return cfg, elapsed, info, kern, state, (t_set, t_get)
+
+
+def tgv_state(cfg, seed=2025):
+    """Taylor-Green vortex on [0, 2 pi]^3: V0 = 1, M = 0.1, p0 = 1/(gamma M^2),
+    p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16, rho = 1 + 1e-3 xi with xi ~ U(-1, 1) from Philox."""
+    g = cfg.gamma
+    n = cfg.nx
+    c = (np.arange(n) + 0.5) * (2 * np.pi / n)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    xi = np.random.Generator(np.random.Philox(seed)).uniform(-1, 1, (n, n, n))
+    rho = 1 + 1e-3 * xi
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    p = 1 / (g * 0.01) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16
+    U = np.zeros((n + 2, n + 2, n + 2, 5))
+    I = U[1:-1, 1:-1, 1:-1]
+    I[..., 0] = rho
+    I[..., 1] = rho * u
+    I[..., 2] = rho * v
+    I[..., 4] = p / (g - 1) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def tgv_3d(n=256, steps=100):
+    """3-D Taylor-Green vortex, periodic, limited linear + HLLC, SSP-RK3, CFL 0.45: FP64 and
+    FP32-residual rates, and the FP32-residual density error against FP64 after the same steps."""
+    out = {"workload": f"3-D Taylor-Green vortex {n}^3 periodic, limited linear (BJ) + HLLC, SSP-RK3, CFL 0.45",
+           "steps": steps}
+    U = None
+    states = {}
+    for prec, name in ((capi.FP64, "fp64"), (capi.FP32_RESIDUAL, "fp32_residual")):
+        cfg = capi.make_config(ndims=3, nx=n, ny=n, nz=n, lx=2 * np.pi, ly=2 * np.pi, lz=2 * np.pi,
+                               periodic=(1, 1, 1), cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                               recon=capi.RECON_LINEAR_LIM, rk_stages=3, precision=prec)
+        if U is None:
+            U = tgv_state(cfg)
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        s.step(2)
+        t0 = time.perf_counter()
+        info = s.step(steps)
+        el = time.perf_counter() - t0
+        bps = s.algorithmic_bytes_per_cell(capi.KIND_ALL)
+        rate = n ** 3 * steps / el
+        out[name] = {"cell_updates_per_s": rate, "ms_per_step": el / steps * 1e3,
+                     "algorithmic_bytes_per_cell_step": bps, "step_roofline_frac": rate * bps / HBM_PEAK,
+                     "final_time": info.time}
+        states[name] = capi.interior(cfg, s.get_state())[..., 0].copy()
+        s.close()
+    d = np.abs(states["fp32_residual"] - states["fp64"])
+    out["density_l1_fp32_vs_fp64"] = float(d.mean())
+    out["density_linf_fp32_vs_fp64"] = float(d.max())
+    return out
 
 
 def cpu_baseline(budget_s=15.0):
@@ -240,6 +297,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -247,7 +305,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     prec = capi.FP64 if args.precision == "fp64" else capi.FP32_RESIDUAL
-    cfg, elapsed, info, kern, state = gpu_run(args.n, args.steps, args.warmup, prec)
+    cfg, elapsed, info, kern, state, (t_set, t_get) = gpu_run(args.n, args.steps, args.warmup, prec)
     ncell = args.n * args.n
     value = ncell * args.steps / elapsed
     dom = max(kern.items(), key=lambda kv: kv[1]["avg_us"] * kv[1]["launches"])
@@ -276,9 +334,22 @@ def main():
         "step_roofline_frac": value * bytes_step / HBM_PEAK,
         "algorithmic_bytes_per_cell_step": bytes_step,
         "kernels": kern,
+        # the same K steps with the host-buffer hand-over of the boundary included (AoS state with
+        # ghosts in via ts_set_state, out via ts_get_state: layout conversion + PCIe copies)
+        "pcie_inclusive_cell_updates_per_s": ncell * args.steps / (t_set + elapsed + t_get),
+        "set_state_s": t_set,
+        "get_state_s": t_get,
         "final_time": info.time,
         "finite": bool(np.isfinite(state).all()),
     }
+    if args.precision == "fp64":
+        # the same run with FP32 residual arithmetic (FP64 state): rate and density error
+        _, el32, _, _, st32, _ = gpu_run(args.n, args.steps, args.warmup, capi.FP32_RESIDUAL)
+        d = np.abs(capi.interior(cfg, st32)[..., 0] - capi.interior(cfg, state)[..., 0])
+        out["fp32_residual"] = {"cell_updates_per_s": ncell * args.steps / el32,
+                                "density_l1_vs_fp64": float(d.mean()), "density_linf_vs_fp64": float(d.max())}
+    if not args.no_3d:
+        out["tgv_3d"] = tgv_3d()
     if not args.no_batch:
         out["batched"] = batched_tubes()
     if not args.no_serve:

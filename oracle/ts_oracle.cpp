@@ -20,6 +20,11 @@
  *   whole step         Sod invariants (mass/energy conservation, momentum growth 0.9*dt),
  *                      first-order convergence to the exact solution, direction invariance
  *                      x/y/z, periodic conservation, diagonal symmetry
+ *   residual norms     first Sod step: norm = 2 h |F*| of the diaphragm face (kernel.h:326)
+ * The exact star state is pinned against the mathematics (f(p*) = 0 to round-off) and Toro's
+ * 5-digit table; the reference's own p* comes from ALGLIB's tolerance-limited LM
+ * (GodunovSolverPerfectGasEOS.h:157-185), so agreement with the reference itself below ~1e-5
+ * relative is parity unpinned.
  * The compiled reference cannot be built in this image without stand-ins for its MSVC MPI
  * header, so no trajectory of the reference itself pins the whole step.
  */

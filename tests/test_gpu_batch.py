@@ -1,4 +1,4 @@
-"""Batched solves (persistent work-queue kernels over many problems): parity with single-solve
+"""Batched solves (persistent kernels over a (problem, tile) table): parity with single-solve
 runs and with the oracle, and the same result alone and in a mix."""
 import numpy as np
 import pytest
@@ -149,3 +149,32 @@ def test_failed_solve_does_not_disturb_the_others():
     assert done_bad and not tickets
     for t, (cfg, _), r in zip(sorted(results), good, ref):
         np.testing.assert_array_equal(capi.interior(cfg, results[t]), capi.interior(cfg, r))
+
+
+def test_batched_stretched_grid_equals_single_solve():
+    """Non-uniform (geometrically stretched) grids go through the batch: their coordinate and size
+    arrays are copied at submission, and the batched solve equals the same solve alone."""
+    n = 300
+    q = 1.01
+    h = q ** np.arange(n + 2)
+    h *= 1.0 / h[1:-1].sum()
+    faces = np.concatenate([[0.0], np.cumsum(h[1:-1])])
+    xc = np.empty(n + 2)
+    xc[1:-1] = 0.5 * (faces[:-1] + faces[1:])
+    xc[0] = xc[1] - h[0]
+    xc[-1] = xc[-2] + 0.5 * (h[-2] + h[-1])
+
+    def make():
+        cfg = capi.make_config(ndims=1, nx=n, cfl=0.45, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                               rk_stages=3, coord=[xc.copy(), None, None], h=[h.copy(), None, None])
+        return cfg, util.sod(cfg)
+
+    cfg, U = make()
+    ref, rinfo = run_alone(cfg, U, 60)
+    b = capi.Batch(hbm_budget_bytes=1 << 26)
+    cfg2, U2 = make()
+    t = b.submit(cfg2, U2, 60)
+    out = drain(b, [t])
+    st, info = out[t]
+    assert info.steps_done == 60 and info.time == rinfo.time
+    np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, ref))

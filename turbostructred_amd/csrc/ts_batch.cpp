@@ -102,6 +102,7 @@ struct Problem {
     int max_steps = 0;
     double t_end = 0;
     std::vector<double> init;  // reference AoS state until admission
+    std::vector<double> coord[3], h[3];  // owned copies of optional non-uniform grid arrays
     size_t bytes = 0, off = 0;
     KArgs desc{};
     int family = -1;
@@ -498,10 +499,7 @@ const char* ts_batch_last_error(const ts_batch* b) { return b ? b->err.c_str() :
 int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state, int max_steps,
                     double t_end, long long* ticket) {
     if (!b || !cfg || !init_state || !ticket || max_steps < 0) return TS_ERR_INVALID;
-    std::string err;
     // the same validation as ts_create, then the batch's own limits
-    ts_ctx* probe = nullptr;
-    (void)probe;
     if (cfg->ndims < 1 || cfg->ndims > 2) return fail(b, TS_ERR_UNSUPPORTED, "batched solves are 1-D or 2-D");
     if (cfg->rk_stages != 3) return fail(b, TS_ERR_UNSUPPORTED, "batched solves use SSP-RK3");
     if (cfg->ghost != 1 || cfg->nx < 2 || (cfg->ndims > 1 && cfg->ny < 2) || cfg->nz != 1 ||
@@ -515,7 +513,19 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
             return fail(b, TS_ERR_INVALID, "side marker refers to a missing boundary condition");
     auto p = std::make_unique<Problem>();
     p->cfg = *cfg;
-    for (int a = 0; a < 3; a++) { p->cfg.coord[a] = nullptr; p->cfg.h[a] = nullptr; }
+    // the caller's grid arrays may be gone by admission time: keep copies
+    for (int a = 0; a < 3; a++) {
+        const int na = (a == 0 ? cfg->nx : a == 1 ? cfg->ny : cfg->nz) + 2;
+        p->cfg.coord[a] = p->cfg.h[a] = nullptr;
+        if (a < cfg->ndims && cfg->coord[a]) {
+            p->coord[a].assign(cfg->coord[a], cfg->coord[a] + na);
+            p->cfg.coord[a] = p->coord[a].data();
+        }
+        if (a < cfg->ndims && cfg->h[a]) {
+            p->h[a].assign(cfg->h[a], cfg->h[a] + na);
+            p->cfg.h[a] = p->h[a].data();
+        }
+    }
     p->n[0] = cfg->nx;
     p->n[1] = cfg->ny;
     p->n[2] = cfg->nz;

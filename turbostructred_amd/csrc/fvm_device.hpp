@@ -100,6 +100,11 @@ __device__ inline double fsqrt(double x) {
 }
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// 2^600 (double) / 2^60 (float): lifts a divisor below 1e-30 into the normal range exactly
+template <typename R> __device__ inline R big_scale();
+template <> __device__ inline double big_scale<double>() { return 0x1p600; }
+template <> __device__ inline float big_scale<float>() { return 0x1p60f; }
+
 template <typename R> __device__ inline R tiny_spread();
 template <> __device__ inline double tiny_spread<double>() { return 3.0 * 2.2250738585072014e-308; }
 template <> __device__ inline float tiny_spread<float>() { return 3.0f * 1.17549435e-38f; }
@@ -174,8 +179,10 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                         }
                         if (m < dmax) {
                             // the divisor can be subnormal (a variable that is zero up to
-                            // round-off): the reciprocal estimate is only used above that range
-                            lim = (dmax > R(1e-30)) ? fdiv(m, dmax) : m / dmax;
+                            // round-off), where the reciprocal estimate fails: both operands
+                            // are then scaled by an exact power of two (0 <= m < dmax)
+                            const R s = (dmax > R(1e-30)) ? R(1) : big_scale<R>();
+                            lim = fdiv(m * s, dmax * s);
                         }
                     } else {
                         R bestA = R(1), bestD = R(1);
@@ -222,14 +229,23 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
         }
     }
     if constexpr (POS && RECON != RC_PWC) {
+        bool bad[2 * ND];
+        bool any = false;
 #pragma unroll
         for (int f = 0; f < 2 * ND; f++) {
             R q = R(2) * face[f][0] * face[f][4] -
                   (face[f][1] * face[f][1] + face[f][2] * face[f][2] + face[f][3] * face[f][3]);
-            if (!(face[f][0] > R(0) && q > R(0))) {
+            bad[f] = !(face[f][0] > R(0) && q > R(0));
+            any = any || bad[f];
+        }
+        // rare: a branch the wavefront skips when none of its lanes needs the fallback
+        if (__builtin_expect(any, 0)) {
 #pragma unroll
-                for (int v = 0; v < NV; v++) face[f][v] = c[v];
-            }
+            for (int f = 0; f < 2 * ND; f++)
+                if (bad[f]) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) face[f][v] = c[v];
+                }
         }
     }
 }
@@ -351,16 +367,20 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
     Fk[1 + D] += p;
     Fk[4] = un * (U[4] + p);
     bool supersonic = left ? (S >= R(0)) : (S <= R(0));
-    R coef = fdiv(m, S - Ss);
+    // supersonic side: the star correction is multiplied by zero (and its operands made finite),
+    // which gives Fk exactly with two selects instead of five
+    R coef = supersonic ? R(0) : fdiv(m, S - Ss);
+    R Sc = supersonic ? R(0) : S;
     R Us[NV];
     Us[0] = coef;
     Us[1] = coef * vel[0];
     Us[2] = coef * vel[1];
     Us[3] = coef * vel[2];
     Us[1 + D] = coef * Ss;
-    Us[4] = coef * (U[4] * ik + (Ss - un) * (Ss + fdiv(p, m)));
+    R pm = supersonic ? R(0) : fdiv(p, m);
+    Us[4] = coef * (U[4] * ik + (Ss - un) * (Ss + pm));
 #pragma unroll
-    for (int i = 0; i < NV; i++) F[i] = supersonic ? Fk[i] : Fk[i] + S * (Us[i] - U[i]);
+    for (int i = 0; i < NV; i++) F[i] = Fk[i] + Sc * (Us[i] - U[i]);
 }
 
 // ---- exact Riemann solver (always double) -----------------------------------------------------

@@ -10,7 +10,7 @@ using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
 // batched (persistent) launch: descriptors, (problem, tile) table, device tile count, number of
 // 256-thread blocks
 using BatchLaunchFn = void (*)(const KArgs*, const int2*, const int*, int, hipStream_t);
-using OccupancyFn = int (*)();  // resident 256-thread blocks per CU of the batched kernels
+using OccupancyFn = int (*)();  // resident workgroups per CU
 
 // dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3, and the
 // fused-dt SSP stages 1 and 2
@@ -20,6 +20,7 @@ struct KernelSet {
     LaunchFn rn = nullptr;  // residual norms of the stencil input (no state written)
     BatchLaunchFn bf1 = nullptr, bf2 = nullptr, bs3 = nullptr;  // fused-dt SSP-RK3, batched
     OccupancyFn bocc = nullptr;
+    OccupancyFn socc = nullptr;  // resident workgroups per CU of the single-solve kernels
 };
 
 // reconstruction variants: 0 PWC, 1 linear, 2 ENO2, 3 limited linear BJ, 4 limited linear Venkat

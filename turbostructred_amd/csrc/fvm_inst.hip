@@ -30,6 +30,30 @@ static void launch(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
 #endif
 }
 
+// resident workgroups per CU of the single-solve stage kernels (the minimum over the kernels of
+// one precision): sets how many tiles run in one round
+template <typename K> static int blocks_per_cu(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+template <int RC, int LM, typename R, typename S, typename E>
+static int single_occupancy() {
+#if TS_DIM == 3
+    const int T = 64 * ZW;
+#define TS_K(MODE, A, B, C) fvm_stage3d<RC, LM, TS_FLUX, MODE, A, B, C>
+#else
+    const int T = 256;
+#define TS_K(MODE, A, B, C) fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, A, B, C>
+#endif
+    int m = blocks_per_cu(TS_K(MODE_F1, R, double, S), T);
+    int v[] = {blocks_per_cu(TS_K(MODE_F2, R, double, S), T), blocks_per_cu(TS_K(MODE_S3, R, S, double), T),
+               blocks_per_cu(TS_K(MODE_DT, R, double, double), T), blocks_per_cu(TS_K(MODE_S1, R, double, E), T)};
+#undef TS_K
+    for (int x : v) m = x < m ? x : m;
+    return m;
+}
+
 #if TS_DIM != 3
 template <int RC, int LM, int MODE, typename R, typename TIn, typename TOut>
 static void launch_batched(const KArgs* d, const int2* t, const int* ntiles, int nblk, hipStream_t s) {
@@ -59,6 +83,7 @@ static void fill(int precision, KernelSet* ks) {
         ks->f1 = launch<RC, LM, MODE_F1, double, double, double>;
         ks->f2 = launch<RC, LM, MODE_F2, double, double, double>;
         ks->rn = launch<RC, LM, MODE_RN, double, double, double>;
+        ks->socc = single_occupancy<RC, LM, double, double, double>;
 #if TS_DIM != 3
         ks->bf1 = launch_batched<RC, LM, MODE_F1, double, double, double>;
         ks->bf2 = launch_batched<RC, LM, MODE_F2, double, double, double>;
@@ -74,6 +99,7 @@ static void fill(int precision, KernelSet* ks) {
         ks->f1 = launch<RC, LM, MODE_F1, float, double, float>;
         ks->f2 = launch<RC, LM, MODE_F2, float, double, float>;
         ks->rn = launch<RC, LM, MODE_RN, float, double, float>;
+        ks->socc = single_occupancy<RC, LM, float, float, double>;
 #if TS_DIM != 3
         ks->bf1 = launch_batched<RC, LM, MODE_F1, float, double, float>;
         ks->bf2 = launch_batched<RC, LM, MODE_F2, float, double, float>;

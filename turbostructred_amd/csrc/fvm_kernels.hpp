@@ -139,6 +139,21 @@ __device__ inline int bc_marker(const GridArgs& g, int side, int face_index) {
     return fm ? int(fm[face_index]) : g.side_marker[side];
 }
 
+// Variable v of an SoA state array at a cell: a wave-uniform 64-bit plane base plus a 32-bit
+// per-lane byte offset -- the global load/store form with a scalar base address, so that each
+// access needs one VGPR of address and no 64-bit address arithmetic.  A plane stays below
+// 4 GiB (checked when a context is created).
+template <typename T>
+__device__ __forceinline__ T ld_plane(const T* base, size_t P, int v, unsigned cell) {
+    const char* b = reinterpret_cast<const char*>(base + (size_t)v * P);
+    return *reinterpret_cast<const T*>(b + cell * unsigned(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st_plane(T* base, size_t P, int v, unsigned cell, T x) {
+    char* b = reinterpret_cast<char*>(base + (size_t)v * P);
+    *reinterpret_cast<T*>(b + cell * unsigned(sizeof(T))) = x;
+}
+
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -278,13 +293,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     auto load_row = [&](int r, R (&u)[NV]) {
         int rs = 0, yside = 0;
         if constexpr (DIM > 1) map_index(r, ny, g.periodic[1], rs, yside);
-        size_t off = (size_t)rs * nx + xs;
+        const unsigned off = unsigned(rs) * unsigned(nx) + unsigned(xs);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(double(uin[v * P + off]) + dt * double(lin[v * P + off]));
+            for (int v = 0; v < NV; v++)
+                u[v] = R(double(ld_plane(uin, P, v, off)) + dt * double(ld_plane(lin, P, v, off)));
         } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+            for (int v = 0; v < NV; v++) u[v] = R(ld_plane(uin, P, v, off));
         }
         if (xside != 0) {
             R t[NV];
@@ -300,18 +316,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     };
 
     // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
-    auto finish = [&](size_t off, const R (&u)[NV], const R (&res)[NV], const double (&un)[NV],
+    auto finish = [&](unsigned off, const R (&u)[NV], const R (&res)[NV], const double (&un)[NV],
                       double ivol, double vol) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             if constexpr (MODE == MODE_RN) {
                 if (out_lane) rn[v] += vol * __builtin_fabs(double(res[v]));
             } else if constexpr (MODE == MODE_F1) {
-                if (out_lane) uo[v * P + off] = TOut(double(res[v]) * ivol);
+                if (out_lane) st_plane(uo, P, v, off, TOut(double(res[v]) * ivol));
             } else {
                 double q = double(u[v]) + dt * (double(res[v]) * ivol);
                 if constexpr (MT::use_un) q = MT::ca * un[v] + MT::cb * q;
-                if (out_lane) uo[v * P + off] = TOut(q);
+                if (out_lane) st_plane(uo, P, v, off, TOut(q));
             }
         }
     };
@@ -323,7 +339,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         load_row(0, c);
         double un[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) un[v] = MT::use_un ? a.un[v * P + xs] : 0.0;
+        for (int v = 0; v < NV; v++) un[v] = MT::use_un ? ld_plane(a.un, P, v, unsigned(xs)) : 0.0;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             nb[0][v] = dpp_from_left(c[v]);
@@ -362,7 +378,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 R Fr = dpp_from_right(F[v]);
                 res[v] = -((Fr - F[v]) * Sx);
             }
-            finish(xs, c, res, un, ihx, hx);
+            finish(unsigned(xs), c, res, un, ihx, hx);
         }
     } else {
         // ---- march along y ----
@@ -370,7 +386,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
         R Um[NV], U0[NV], Up[NV];
         R Pn[NV];               // prefetched stencil row (consumed as Up one iteration later)
-        double Nc[NV], Nn[NV];  // U^n of the row finalized in this / the next iteration
+        double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
         R ytop[NV];             // +y face state of the previously reconstructed row
         R gbot[NV];             // y-flux through the bottom face of the pending row
         R dfx[NV];              // x part of the residual of the pending row
@@ -380,18 +396,21 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         load_row(y0 - 1, U0);
         load_row(y0, Pn);
 #pragma unroll
-        for (int v = 0; v < NV; v++) Nn[v] = 0.0;
+        for (int v = 0; v < NV; v++) Nc[v] = 0.0;
         const int nit = yend - y0 + 2;
         for (int t = 0; t < nit; t++) {
             const int rc = y0 - 1 + t;  // row being reconstructed
 #pragma unroll
-            for (int v = 0; v < NV; v++) { Up[v] = Pn[v]; Nc[v] = Nn[v]; }
+            for (int v = 0; v < NV; v++) Up[v] = Pn[v];
             if (t + 1 < nit) load_row(rc + 2, Pn);
             if constexpr (MT::use_un) {
-                if (rc >= y0 && rc < yend) {
-                    const size_t offn = (size_t)rc * nx + xs;
+                // U^n of the row finalized below, loaded now: the reconstruction and the y-face
+                // flux cover its latency (it was read three iterations ago as a stencil row, so
+                // it is normally an L2 hit), and only one buffer is live
+                if (t >= 2) {
+                    const unsigned offn = unsigned(rc - 1) * unsigned(nx) + unsigned(xs);
 #pragma unroll
-                    for (int v = 0; v < NV; v++) Nn[v] = a.un[v * P + offn];
+                    for (int v = 0; v < NV; v++) Nc[v] = ld_plane(a.un, P, v, offn);
                 }
             }
             int rcs, rcside;
@@ -444,7 +463,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                         R res[NV];
 #pragma unroll
                         for (int v = 0; v < NV; v++) res[v] = dfx[v] - (G[v] - gbot[v]) * Sy;
-                        finish((size_t)rf * nx + xs, Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
+                        finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
                     }
 #pragma unroll
                     for (int v = 0; v < NV; v++) gbot[v] = G[v];

@@ -79,13 +79,14 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         int rs, yside, kks, zside;
         map_index(r, ny, g.periodic[1], rs, yside);
         map_index(kk, nz, g.periodic[2], kks, zside);
-        size_t off = ((size_t)kks * ny + rs) * nx + xs;
+        const unsigned off = (unsigned(kks) * unsigned(ny) + unsigned(rs)) * unsigned(nx) + unsigned(xs);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(double(uin[v * P + off]) + dt * double(lin[v * P + off]));
+            for (int v = 0; v < NV; v++)
+                u[v] = R(double(ld_plane(uin, P, v, off)) + dt * double(ld_plane(lin, P, v, off)));
         } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(uin[v * P + off]);
+            for (int v = 0; v < NV; v++) u[v] = R(ld_plane(uin, P, v, off));
         }
         int axis = -1, side = 0, fidx = 0;
         if (xside != 0) { axis = 0; side = xside < 0 ? 0 : 1; fidx = kks * ny + rs; }
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
     R Um[NV], U0[NV], Up[NV], Pn[NV];
     R Zx[NV];               // centre row of the z-neighbour no wavefront holds (edge wavefronts)
-    double Nc[NV], Nn[NV];  // U^n of the row finalized in this / the next iteration
+    double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
     R ytop[NV];             // +y face state of the previous row
     R gbot[NV];             // y-flux through the bottom face of the pending row
     R dfx[NV], dhz[NV];     // x and z parts of the residual of the pending row
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     load_cell(y0 - 1, k, U0);
     load_cell(y0, k, Pn);
 #pragma unroll
-    for (int v = 0; v < NV; v++) { Nn[v] = 0.0; dfx[v] = dhz[v] = gbot[v] = R(0); }
+    for (int v = 0; v < NV; v++) { Nc[v] = 0.0; dfx[v] = dhz[v] = gbot[v] = R(0); }
     const bool edge = (w == 0) || (w == ZW - 1);
     const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
     const int nit = yend - y0 + 2;
@@ -124,14 +125,15 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         const int rc = y0 - 1 + t;  // row being reconstructed
         const bool row_out = rc >= y0 && rc < yend;
 #pragma unroll
-        for (int v = 0; v < NV; v++) { Up[v] = Pn[v]; Nc[v] = Nn[v]; }
+        for (int v = 0; v < NV; v++) Up[v] = Pn[v];
         if (t + 1 < nit) load_cell(rc + 2, k, Pn);
         if (edge) load_cell(rc, k + zoff, Zx);
         if constexpr (MT::use_un) {
-            if (out_plane && row_out) {
-                const size_t offn = ((size_t)ks * ny + rc) * nx + xs;
+            // U^n of the row finalized in this iteration (read earlier as a stencil row)
+            if (out_plane && t >= 2) {
+                const unsigned offn = (unsigned(ks) * unsigned(ny) + unsigned(rc - 1)) * unsigned(nx) + unsigned(xs);
 #pragma unroll
-                for (int v = 0; v < NV; v++) Nn[v] = a.un[v * P + offn];
+                for (int v = 0; v < NV; v++) Nc[v] = ld_plane(a.un, P, v, offn);
             }
         }
         int rcs, rcside;
@@ -235,18 +237,18 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                     // finalize row rc-1 (held in Um)
                     const int rf = rc - 1;
                     const double ivol = ihx * g.ih[1][rf + 1] * ihz;
-                    const size_t off = ((size_t)ks * ny + rf) * nx + xs;
+                    const unsigned off = (unsigned(ks) * unsigned(ny) + unsigned(rf)) * unsigned(nx) + unsigned(xs);
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         R res = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
                         if constexpr (MODE == MODE_RN) {
                             if (out_lane) rn[v] += hx * g.h[1][rf + 1] * hz * __builtin_fabs(double(res));
                         } else if constexpr (MODE == MODE_F1) {
-                            if (out_lane) uo[v * P + off] = TOut(double(res) * ivol);
+                            if (out_lane) st_plane(uo, P, v, off, TOut(double(res) * ivol));
                         } else {
                             double q = double(Um[v]) + dt * (double(res) * ivol);
                             if constexpr (MT::use_un) q = MT::ca * Nc[v] + MT::cb * q;
-                            if (out_lane) uo[v * P + off] = TOut(q);
+                            if (out_lane) st_plane(uo, P, v, off, TOut(q));
                         }
                     }
                 }

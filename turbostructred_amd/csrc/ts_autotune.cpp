@@ -31,19 +31,38 @@ Key key_of(const ts_config& c) {
 }  // namespace
 
 namespace tsr {
-int tuned_rows_per_tile(const ts_config& c) {
+int tuned_rows_per_tile(const ts_config& c, long long slots) {
     if (c.ndims == 1) return 1;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_table.find(key_of(c));
         if (it != g_table.end()) return it->second;
     }
-    // default: enough tiles to fill 256 CUs several times over, at most 32 rows
-    long long strips = (c.nx + 59) / 60;
-    long long rows = c.ndims == 2 ? c.ny : (long long)c.ny * ((c.nz + 5) / 6);
-    int ty = 32;
-    while (ty > 4 && strips * ((rows + ty - 1) / ty) < 4096) ty /= 2;
-    return ty;
+    const long long strips = (c.nx + 59) / 60;
+    const long long zb = c.ndims == 3 ? (c.nz + 5) / 6 : 1;  // z blocks of a 3-D tile column
+    if (slots <= 0) {
+        // no occupancy known: enough tiles to fill 256 CUs several times over, at most 32 rows
+        long long rows = (long long)c.ny * zb;
+        int ty = 32;
+        while (ty > 4 && strips * ((rows + ty - 1) / ty) < 4096) ty /= 2;
+        return ty;
+    }
+    // A tile is one wavefront's (3-D: one workgroup's) march of ty rows plus two warm-up rows,
+    // and tiles run in rounds of `slots`.  The time goes as rounds * (ty + 2): pick the ty that
+    // minimises it, so that the last round is not a mostly empty one.
+    int best = 4;
+    double best_cost = 1e300;
+    const int hi = c.ny < 256 ? c.ny : 256;
+    for (int ty = 4; ty <= (hi > 4 ? hi : 4); ty++) {
+        long long tiles = strips * zb * ((c.ny + ty - 1) / ty);
+        long long rounds = (tiles + slots - 1) / slots;
+        double cost = double(rounds) * (ty + 2);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = ty;
+        }
+    }
+    return best;
 }
 }  // namespace tsr
 

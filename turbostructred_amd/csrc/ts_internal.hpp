@@ -73,7 +73,8 @@ bool uniform_axis(const std::vector<double>& H);
 int recon_variant(const ts_config& c);
 bool lookup_kernels(int dim, int flux, int rv, int precision, tsd::KernelSet* ks);
 void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, double* o);
-int tuned_rows_per_tile(const ts_config& c);
+// output rows per tile; `slots` = tiles the GPU holds at once (0: unknown)
+int tuned_rows_per_tile(const ts_config& c, long long slots = 0);
 }  // namespace tsr
 
 size_t ts_state_size_cfg(const ts_config* c);

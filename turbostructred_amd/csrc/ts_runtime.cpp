@@ -65,6 +65,11 @@ int validate(const ts_config* c, std::string& err) {
         if (n[a] < 2) { err = "every active axis needs at least 2 cells"; return TS_ERR_INVALID; }
     for (int a = c->ndims; a < 3; a++)
         if (n[a] != 1) { err = "inactive axes must have 1 cell"; return TS_ERR_INVALID; }
+    // a state plane (one variable, one double per cell) is addressed with 32-bit byte offsets
+    if ((long long)n[0] * n[1] * n[2] * 8 >= (1ll << 32)) {
+        err = "more than 2^29 cells: a state plane would exceed 4 GiB";
+        return TS_ERR_UNSUPPORTED;
+    }
     if (!(c->gamma > 1.0)) { err = "gamma must exceed 1"; return TS_ERR_INVALID; }
     if (!(c->cfl > 0.0)) { err = "CFL must be positive"; return TS_ERR_INVALID; }
     if (c->riemann != TS_RIEMANN_ROE && c->riemann != TS_RIEMANN_GODUNOV && c->riemann != TS_RIEMANN_HLLC) {

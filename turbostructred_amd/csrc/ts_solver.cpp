@@ -52,6 +52,13 @@ inline size_t gidx(const ts_ctx* c, int i, int j, int k) {
 
 constexpr int CHUNK = 256;  // steps issued between host checks of the termination flag
 
+// tiles resident at once: CUs x resident workgroups per CU x tiles per workgroup
+long long tile_slots(const ts_ctx* c, int tiles_per_block) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->cfg.device) != hipSuccess || !c->ks.socc) return 0;
+    return (long long)prop.multiProcessorCount * c->ks.socc() * tiles_per_block;
+}
+
 }  // namespace
 
 int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
@@ -163,14 +170,14 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->block_dim = dim3(256);
         c->grid_dim = dim3((c->ntiles + 3) / 4);
     } else if (c->nd == 2) {
-        c->ty = tsr::tuned_rows_per_tile(*cfg);
+        c->ty = tsr::tuned_rows_per_tile(*cfg, tile_slots(c, 4));
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
         int nyb = (c->n[1] + c->ty - 1) / c->ty;
         c->ntiles = c->nstrips * nyb;
         c->block_dim = dim3(256);
         c->grid_dim = dim3((c->ntiles + 3) / 4);
     } else {
-        c->ty = tsr::tuned_rows_per_tile(*cfg);
+        c->ty = tsr::tuned_rows_per_tile(*cfg, tile_slots(c, 1));
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
         int nyb = (c->n[1] + c->ty - 1) / c->ty;
         int nzb = (c->n[2] + ZOUT - 1) / ZOUT;

@@ -309,7 +309,8 @@ int admit(ts_batch* b, Problem& p) {
     d.uout = nullptr;
     d.plane = n;
     d.ctl = (StepCtl*)(base + L.ctl);
-    d.ty = c.ndims == 1 ? 1 : tsr::tuned_rows_per_tile(c);
+    // tile height: set for the whole family when its tables are built (retile)
+    d.ty = c.ndims == 1 ? 1 : 4;
     d.nstrips = (c.nx + STRIP - 1) / STRIP;
     d.ntiles = d.nstrips * (c.ndims == 1 ? 1 : (c.ny + d.ty - 1) / d.ty);
     p.ntiles = d.ntiles;
@@ -331,7 +332,36 @@ KArgs stage_desc(const Problem& p, int kernel, char* arena) {
     return a;
 }
 
+// 2-D tile height of a family, chosen again whenever its membership changes (results do not
+// depend on the tiling): the tallest of 32/16/8/4 rows that still gives a tile to at least half
+// of the persistent grid's wavefronts.  Under load, tall tiles spend less on the two warm-up
+// rows and the per-tile dt atomics; a lightly loaded batch keeps short tiles so that each step
+// stays short.
+void retile(ts_batch* b, Family& f) {
+    if (std::get<0>(f.key) < 2) return;
+    const long long waves = (long long)b->ncu * f.occupancy * 4 / 2;
+    int ty = 4;
+    for (int cand = 32; cand > 4; cand /= 2) {
+        long long tiles = 0;
+        for (long long t : f.members) {
+            const Problem& p = *b->probs[t];
+            tiles += (long long)p.desc.nstrips * ((p.n[1] + cand - 1) / cand);
+        }
+        if (tiles >= waves) {
+            ty = cand;
+            break;
+        }
+    }
+    for (long long t : f.members) {
+        Problem& p = *b->probs[t];
+        p.desc.ty = ty;
+        p.desc.ntiles = p.desc.nstrips * ((p.n[1] + ty - 1) / ty);
+        p.ntiles = p.desc.ntiles;
+    }
+}
+
 int rebuild_tables(ts_batch* b, Family& f) {
+    retile(b, f);
     size_t np = f.members.size();
     size_t nt = 0;
     for (long long t : f.members) nt += b->probs[t]->ntiles;

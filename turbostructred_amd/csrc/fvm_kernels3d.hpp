@@ -5,8 +5,9 @@
 // loads the one extra centre row its outer z-neighbour needs).  Inside a plane the scheme is the
 // 2-D one: 64 lanes along x (60 output columns) and a register window marching along y.
 // z-neighbours travel through LDS once per row: the centre row of the state, then the z+ face
-// state of each plane, then the z-face flux -- three barriers per row.  Each z-face flux is
-// evaluated once, by the wavefront above it.  The residual of a cell is accumulated x, then y,
+// state of each plane, then the z-face flux, which is read one row later after the next row's
+// first barrier -- two barriers per row.  Each z-face flux is evaluated once, by the wavefront
+// above it.  The residual of a cell is accumulated x, then y,
 // then z (ERK:1232,1312,1392) and its spectral radius x-, x+, y-, y+, z-, z+ (ERK:1225-1248,
 // 1305-1328, 1385-1408).
 #pragma once
@@ -120,6 +121,10 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     for (int v = 0; v < NV; v++) { Nc[v] = 0.0; dfx[v] = dhz[v] = gbot[v] = R(0); }
     const bool edge = (w == 0) || (w == ZW - 1);
     const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
+    R Hc[NV], lamzc = R(0), Szc = R(0);  // z-flux below this plane for the previous row
+#pragma unroll
+    for (int v = 0; v < NV; v++) Hc[v] = R(0);
+    bool prev_out = false;
     const int nit = yend - y0 + 2;
     for (int t = 0; t < nit; t++) {
         const int rc = y0 - 1 + t;  // row being reconstructed
@@ -148,6 +153,17 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; v++) lds_u[w][v][lane] = U0[v];
         __syncthreads();
+        // (3) one row late: the flux through the face above the previous row, which the
+        // wavefront above wrote before this barrier; it is read before the next barrier, after
+        // which lds_h is written again (two barriers per row instead of three)
+        if (out_plane && prev_out) {
+            lzb = lamzc;
+            lzt = lds_l[w + 1][lane];
+            if constexpr (MT::flux) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) dhz[v] = (lds_h[w + 1][v][lane] - Hc[v]) * Szc;
+            }
+        }
         R face[6][NV];
         {
             R nb[6][NV];
@@ -186,18 +202,11 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             }
             lds_l[w][lane] = lamz;
         }
-        // (3) the flux through the face above comes from the wavefront above
-        __syncthreads();
-        R dhz_new[NV], lzb_new = lamz, lzt_new = R(0);
 #pragma unroll
-        for (int v = 0; v < NV; v++) dhz_new[v] = R(0);
-        if (out_plane && row_out) {
-            lzt_new = lds_l[w + 1][lane];
-            if constexpr (MT::flux) {
-#pragma unroll
-                for (int v = 0; v < NV; v++) dhz_new[v] = (lds_h[w + 1][v][lane] - H[v]) * Sz;
-            }
-        }
+        for (int v = 0; v < NV; v++) Hc[v] = H[v];
+        lamzc = lamz;
+        Szc = Sz;
+        prev_out = row_out;
         if constexpr (MT::sr) {
             // top ghost plane: only the face below it (ERK:1385)
             if (row_out && k == nz && w >= 1 && x_out)
@@ -290,11 +299,6 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                 }
             }
         }
-        // the z part of the row just reconstructed becomes the pending row's
-#pragma unroll
-        for (int v = 0; v < NV; v++) dhz[v] = dhz_new[v];
-        lzb = lzb_new;
-        lzt = lzt_new;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             ytop[v] = face[3][v];

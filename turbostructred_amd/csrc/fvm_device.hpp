@@ -390,18 +390,22 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
 static __device__ __noinline__ double dpow(double x, double y) { return pow(x, y); }
 
 // f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37).
-__device__ inline double fk_dev(double g, double r, double pk, double ak, double p, double& d) {
+__device__ inline double fk_dev(double g, double r, double pk, double ak, double p, double& d, double& pz) {
     if (p > pk) {
         double A = 2.0 / ((g + 1.0) * r);
         double B = pk * (g - 1.0) / (g + 1.0);
         double q = __builtin_sqrt(A / (p + B));
         d = q * (1.0 - 0.5 * (p - pk) / (B + p));
+        pz = 0.0;
         return (p - pk) * q;
     }
+    // rarefaction: one power, (p/pK)^z with z = (g-1)/2g, gives f, f' = (p/pK)^(z-1)/(r aK) and
+    // the tail speed of the fan (the sampler reuses pz)
     double pr = p / pk;
-    double res = dpow(pr, (g - 1.0) / (2.0 * g)) - 1.0;
+    pz = dpow(pr, (g - 1.0) / (2.0 * g));
+    double res = pz - 1.0;
     res *= 2.0 * ak / (g - 1.0);
-    d = dpow(pr, -(g + 1.0) / (2.0 * g)) / (r * ak);
+    d = pz / (pr * r * ak);
     return res;
 }
 
@@ -416,6 +420,7 @@ struct StarDev {
 __device__ inline StarDev star_state_dev(double g, double rl, double ul, double pl, double rr,
                                          double ur, double pr) {
     StarDev s;
+    double pe = -1.0, fle = 0.0, fre = 0.0, zle = 0.0, zre = 0.0;  // last Newton evaluation
     double al = __builtin_sqrt(g * pl / rl), ar = __builtin_sqrt(g * pr / rr);
     double du = ur - ul;
     s.vacuum = (2.0 / (g - 1.0)) * (al + ar) <= du;
@@ -448,8 +453,10 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         }
         if (!(p > 0.0)) p = 1e-12 * pmin;
         for (int it = 1; it <= 60; it++) {
-            double dl, dr;
-            double f = fk_dev(g, rl, pl, al, p, dl) + fk_dev(g, rr, pr, ar, p, dr) + du;
+            double dl, dr, zl, zr;
+            double fl = fk_dev(g, rl, pl, al, p, dl, zl), fr = fk_dev(g, rr, pr, ar, p, dr, zr);
+            pe = p, fle = fl, fre = fr, zle = zl, zre = zr;
+            double f = fl + fr + du;
             double pn = p - f / (dl + dr);
             if (!(pn > 0.0)) pn = 0.01 * p;
             double ch = __builtin_fabs(pn - p) / (pn + p);
@@ -458,8 +465,14 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         }
     }
     s.p = p;
-    double d1, d2;
-    double f1 = fk_dev(g, rl, pl, al, p, d1), f2 = fk_dev(g, rr, pr, ar, p, d2);
+    double f1, f2, pzl, pzr;
+    if (p == pe) {  // Newton stopped on a fixed point (e.g. equal states): f_L, f_R are known
+        f1 = fle, f2 = fre, pzl = zle, pzr = zre;
+    } else {
+        double d1, d2;
+        f1 = fk_dev(g, rl, pl, al, p, d1, pzl);
+        f2 = fk_dev(g, rr, pr, ar, p, d2, pzr);
+    }
     s.u = 0.5 * (ul + ur) + 0.5 * (f2 - f1);
     double C = (g - 1.0) / (g + 1.0);
     s.smax = 0.0;
@@ -474,7 +487,7 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         s.shock_l = false;
         s.rl = rl * dpow(prl, 1.0 / g);
         s.SHL = ul - al;
-        s.STL = s.u - al * dpow(prl, (g - 1) / (2 * g));
+        s.STL = s.u - al * pzl;  // (p/pL)^((g-1)/2g), from f_L
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHL));
     }
     double prr = p / pr;
@@ -487,7 +500,7 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         s.shock_r = false;
         s.rr = rr * dpow(prr, 1.0 / g);
         s.SHR = ur + ar;
-        s.STR = s.u + ar * dpow(prr, (g - 1) / (2 * g));
+        s.STR = s.u + ar * pzr;
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHR));
     }
     if (s.vacuum) {  // the fan tails bound the vacuum (Toro sec. 4.6)

@@ -613,8 +613,21 @@ __global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict
             if (!ctl->go) continue;
         }
         double m = __builtin_inf();
-        stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, e.y, lane, m);
-        if constexpr (MT::sr) wave_reduce_finalize(ctl, a.g.cfl, m, lane, a.ntiles);
+        int nwork = a.ntiles;
+        if constexpr (DIM == 1) {
+            // a 1-D work item is a group of a.ty consecutive strips, walked by one wavefront
+            const int s0 = e.y * a.ty;
+            const int s1 = (s0 + a.ty < a.ntiles) ? s0 + a.ty : a.ntiles;
+            for (int sidx = s0; sidx < s1; sidx++) {
+                double ms = __builtin_inf();
+                stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, sidx, lane, ms);
+                m = __builtin_fmin(m, ms);
+            }
+            nwork = (a.ntiles + a.ty - 1) / a.ty;
+        } else {
+            stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, e.y, lane, m);
+        }
+        if constexpr (MT::sr) wave_reduce_finalize(ctl, a.g.cfl, m, lane, nwork);
     }
 }
 

@@ -332,14 +332,33 @@ KArgs stage_desc(const Problem& p, int kernel, char* arena) {
     return a;
 }
 
-// 2-D tile height of a family, chosen again whenever its membership changes (results do not
-// depend on the tiling): the tallest of 32/16/8/4 rows that still gives a tile to at least half
+// Tiling of a family, chosen again whenever its membership changes (results do not depend on
+// the tiling).  2-D: the tallest of 32/16/8/4 rows that still gives a tile to at least half
 // of the persistent grid's wavefronts.  Under load, tall tiles spend less on the two warm-up
 // rows and the per-tile dt atomics; a lightly loaded batch keeps short tiles so that each step
 // stays short.
 void retile(ts_batch* b, Family& f) {
-    if (std::get<0>(f.key) < 2) return;
     const long long waves = (long long)b->ncu * f.occupancy * 4 / 2;
+    if (std::get<0>(f.key) == 1) {
+        // 1-D: a work item is a group of G consecutive 60-cell strips (G = 8/4/2/1, the largest
+        // that still gives half of the wavefronts an item): fewer descriptor loads and dt atomics
+        int G = 1;
+        for (int cand = 8; cand > 1; cand /= 2) {
+            long long items = 0;
+            for (long long t : f.members) items += (b->probs[t]->desc.nstrips + cand - 1) / cand;
+            if (items >= waves) {
+                G = cand;
+                break;
+            }
+        }
+        for (long long t : f.members) {
+            Problem& p = *b->probs[t];
+            p.desc.ty = G;
+            p.desc.ntiles = p.desc.nstrips;
+            p.ntiles = (p.desc.nstrips + G - 1) / G;
+        }
+        return;
+    }
     int ty = 4;
     for (int cand = 32; cand > 4; cand /= 2) {
         long long tiles = 0;

@@ -85,24 +85,25 @@ struct KArgs {
 };
 
 // ---- wave-level helpers -------------------------------------------------------------------------
-// lane i receives lane i-1 (wave_shr:1) / lane i+1 (wave_shl:1); the end lanes are halo lanes.
+// lane i receives lane i-1 (wave_shr:1) / lane i+1 (wave_shl:1); the end lanes are halo lanes
+// and receive 0 (bound_ctrl), which needs no copy of an "old" value before each move.
 __device__ inline double dpp_from_left(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 __device__ inline double dpp_from_right(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 __device__ inline float dpp_from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ inline float dpp_from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of

@@ -88,3 +88,56 @@ def test_driver_matches_oracle(tmp_path, case):
     if norms:
         r = np.array(info.residual_norm)
         assert np.all(np.abs(d["norms"] - r) <= 1e-12 * np.abs(r).max())
+
+
+def read_sol(path, nx):
+    """A one-rank .sol file (kernel.h:992-1029): double time, int rank, then 5 doubles per interior
+    cell in k, j, i order."""
+    raw = open(path, "rb").read()
+    assert len(raw) == 8 + 4 + nx * 5 * 8, len(raw)
+    t = np.frombuffer(raw[:8], dtype=np.float64)[0]
+    rank = np.frombuffer(raw[8:12], dtype=np.int32)[0]
+    return t, int(rank), np.frombuffer(raw[12:], dtype=np.float64).reshape(nx, 5)
+
+
+def test_save_solution_file_matches_oracle(tmp_path):
+    """SaveSolution of the C++ host class writes the reference's one-rank .sol layout; read back
+    independently, it holds the oracle's time and state."""
+    T = TORO["1"]
+    nx = 200
+    sol = str(tmp_path / "a.sol")
+    out = str(tmp_path / "out.bin")
+    subprocess.run([DRIVER, *map(repr, T["left"]), *map(repr, T["right"]), repr(T["x0"]), repr(T["t"]), str(nx),
+                    str(capi.RECON_LINEAR_LIM), str(capi.RIEMANN_HLLC), "3", "0", "0.0", "0", out, sol, "-"],
+                   check=True, timeout=120)
+    t, rank, cells = read_sol(sol, nx)
+    cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=3, bcs=[dict(type=capi.BC_NATURAL)])
+    xc = (np.arange(nx) + 0.5) / nx
+    left = xc < T["x0"]
+    U = np.zeros((1, 1, nx + 2, 5))
+    U[0, 0, 1:-1] = util.prim2cons(np.where(left, T["left"][0], T["right"][0]), np.where(left, T["left"][1], T["right"][1]),
+                                   0 * xc, 0 * xc, np.where(left, T["left"][2], T["right"][2]), 1.4)
+    o, info, _, _ = run_oracle(cfg, U, T["t"], 0.0)
+    assert rank == 0 and abs(t - info.time) <= 1e-12 * info.time
+    err = util.normwise_rel(cells.reshape(1, 1, nx, 5), capi.interior(cfg, o.get_state()))
+    assert np.all(err <= 1e-12), err
+
+
+def test_restart_from_solution_file_is_seamless(tmp_path):
+    """LoadSolution (kernel.h:1031-1079) then Run: a run stopped at T/2, saved and restarted to T
+    ends bitwise where the uninterrupted run (snapshot-clamped at T/2) ends."""
+    T = TORO["3"]
+    nx, half = 240, T["t"] / 2
+    common = [*map(repr, T["left"]), *map(repr, T["right"]), repr(T["x0"])]
+    method = [str(nx), str(capi.RECON_LINEAR_LIM), str(capi.RIEMANN_HLLC), "3", "0"]
+    a_sol, b_sol, c_sol = (str(tmp_path / f"{n}.sol") for n in "abc")
+    out = str(tmp_path / "out.bin")
+    subprocess.run([DRIVER, *common, repr(T["t"]), *method, repr(half), "0", out, a_sol, "-"], check=True, timeout=120)
+    subprocess.run([DRIVER, *common, repr(half), *method, repr(half), "0", out, b_sol, "-"], check=True, timeout=120)
+    subprocess.run([DRIVER, *common, repr(T["t"]), *method, repr(half), "0", out, c_sol, b_sol], check=True, timeout=120)
+    ta, _, a = read_sol(a_sol, nx)
+    tb, _, _ = read_sol(b_sol, nx)
+    tc, _, c = read_sol(c_sol, nx)
+    assert tb == half and ta == tc
+    np.testing.assert_array_equal(a, c)

@@ -4,15 +4,17 @@
 // SetInitialConditions, Run.
 //
 // usage: shock_tube_driver rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress
-//                          save_time residual_norms out.bin
+//                          save_time residual_norms out.bin [save.sol load.sol]
 //   recon / riemann: the reference's enum values; rk: 1 or 3; compress: 1 = the two-block
 //   geometric grid of RunSODTestRoe1D (q = 1/1.05 then 1.05, node at 0.5 L), 0 = uniform;
-//   save_time > 0 clamps dt onto its multiples (SaveSolutionTime).
+//   save_time > 0 clamps dt onto its multiples (SaveSolutionTime); save.sol / load.sol ("-" for
+//   none): Kernel::SaveSolution after the run / Kernel::LoadSolution before it (the restart path).
 // out.bin: int32 nXAll, int32 snapshots, int32 iteration, pad; double time, dt, residual[5];
 //          CoordinateX[nXAll], hx[nXAll], values[nXAll*5].
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <string>
 #include <vector>
 
 #include "ts_kernel.hpp"
@@ -20,11 +22,14 @@
 using namespace tsh;
 
 int main(int argc, char** argv) {
-    if (argc != 17) {
-        std::fprintf(stderr, "usage: %s rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress save_time norms out.bin\n",
+    if (argc != 17 && argc != 19) {
+        std::fprintf(stderr,
+                     "usage: %s rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress save_time norms out.bin"
+                     " [save.sol load.sol]\n",
                      argv[0]);
         return 2;
     }
+    const std::string save_sol = argc == 19 ? argv[17] : "-", load_sol = argc == 19 ? argv[18] : "-";
     const double roL = std::atof(argv[1]), uL = std::atof(argv[2]), PL = std::atof(argv[3]);
     const double roR = std::atof(argv[4]), uR = std::atof(argv[5]), PR = std::atof(argv[6]);
     const double x0 = std::atof(argv[7]), t_end = std::atof(argv[8]);
@@ -73,10 +78,12 @@ int main(int argc, char** argv) {
             const double ro = left ? roL : roR, u = left ? uL : uR, p = left ? PL : PR;
             return std::vector<double>{ro, ro * u, 0.0, 0.0, p / g1 + 0.5 * ro * u * u};
         });
+        if (load_sol != "-") kernel->LoadSolution(load_sol);
         int snapshots = 0;
         kernel->OnSolutionSnapshot = [&](HipExplicitRungeKuttaFVM&) { snapshots++; };
         kernel->Run();
         kernel->SyncValues();
+        if (save_sol != "-") kernel->SaveSolution(save_sol);
 
         FILE* f = std::fopen(out, "wb");
         if (!f) throw std::runtime_error("cannot open output file");

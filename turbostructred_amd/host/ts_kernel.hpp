@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <memory>
@@ -348,10 +349,55 @@ class HipExplicitRungeKuttaFVM {
         }
     }
 
+    // Kernel::SaveSolution (kernel.h:992-1029), as one rank writes it: the time (double), the
+    // rank (int 0), then every interior cell's 5 doubles in k, j, i order (i fastest).  The
+    // state comes from the device first.
+    void SaveSolution(const std::string& fname) {
+        SyncValues();
+        std::FILE* f = std::fopen(fname.c_str(), "wb");
+        if (!f) throw std::runtime_error("cannot open " + fname + " for writing");
+        const int rank = 0;
+        bool ok = std::fwrite(&stepInfo.Time, sizeof(double), 1, f) == 1 && std::fwrite(&rank, sizeof(int), 1, f) == 1;
+        ForInterior([&](size_t s) { ok = ok && std::fwrite(&values[s], sizeof(double), nVariables, f) == (size_t)nVariables; });
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) throw std::runtime_error("writing " + fname + " failed");
+    }
+
+    // Kernel::LoadSolution (kernel.h:1031-1079) for a file written by one rank: restores the
+    // time and the interior state (ghosts are regenerated on the device), and moves the next
+    // snapshot times past the restored time as kernel.h:1043-1044 do.  The iteration counter
+    // is not stored in the format and keeps its value.
+    void LoadSolution(const std::string& fname) {
+        std::FILE* f = std::fopen(fname.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open " + fname);
+        double t = 0;
+        int rank = -1;
+        bool ok = std::fread(&t, sizeof(double), 1, f) == 1 && std::fread(&rank, sizeof(int), 1, f) == 1;
+        ok = ok && rank == 0;
+        ForInterior([&](size_t s) { ok = ok && std::fread(&values[s], sizeof(double), nVariables, f) == (size_t)nVariables; });
+        char extra;
+        ok = ok && std::fread(&extra, 1, 1, f) == 0;  // exactly one rank's block: nothing left over
+        std::fclose(f);
+        if (!ok) throw std::runtime_error(fname + " is not a one-rank solution file of this grid");
+        stepInfo.Time = t;
+        if (SaveSolutionTime > 0) stepInfo.NextSolutionSnapshotTime = (int(t / SaveSolutionTime) + 1) * SaveSolutionTime;
+        if (SaveSliceTime > 0) stepInfo.NextSliceSnapshotTime = (int(t / SaveSliceTime) + 1) * SaveSliceTime;
+        UploadValues();
+        check(ts_set_time(ctx_, stepInfo.Time, stepInfo.Iteration));
+    }
+
     ts_ctx* Context() const { return ctx_; }
 
    private:
     ts_ctx* ctx_ = nullptr;
+    // visit the interior cells in the reference's k, j, i order; s = offset in `values`
+    template <typename F> void ForInterior(F&& f) {
+        const int gy = nDims > 1 ? 1 : 0, gz = nDims > 2 ? 1 : 0;
+        for (int k = 0; k < nZ; k++)
+            for (int j = 0; j < nY; j++)
+                for (int i = 0; i < nX; i++)
+                    f((((size_t)(k + gz) * nYAll + (j + gy)) * nXAll + (i + 1)) * nVariables);
+    }
     bool norms_on_ = false;
     std::vector<uint8_t> face_markers_[6];
 

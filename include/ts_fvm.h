@@ -134,6 +134,12 @@ size_t ts_state_size(const ts_ctx* ctx);
 int ts_set_state(ts_ctx* ctx, const double* aos_values, size_t n_doubles);
 int ts_get_state(ts_ctx* ctx, double* aos_values, size_t n_doubles);
 
+/* Values of selected interior cells, without downloading the whole state: the 5 conserved
+   values of cell (ijk[3c], ijk[3c+1], ijk[3c+2]) go to out[5c..5c+4].  Indices are the
+   reference's local ones (ghost layer included, as Kernel::getCellValues, kernel.h:93-96): i in
+   1..nx, j in 1..ny (0 below 2-D), k in 1..nz (0 below 3-D).  For sensors (src/Sensors/Sensor.h). */
+int ts_get_cells(ts_ctx* ctx, const int* ijk, size_t n, double* out);
+
 /* Restore Time and Iteration after LoadSolution (kernel.h:1031-1079). */
 int ts_set_time(ts_ctx* ctx, double time, long long iteration);
 
@@ -148,7 +154,8 @@ int ts_step(ts_ctx* ctx, int max_steps, double t_max, double next_snapshot_time,
 
 /* Enable the residual norms sum(V*|R|) per variable of UpdateSolution (kernel.h:326), reported
    in ts_step_info.residual_norm for the last step taken (R of U^n: the forward-Euler residual,
-   or SSP-RK3 stage 1).  Off by default: they cost one extra residual pass per step. */
+   or SSP-RK3 stage 1).  Off by default: they cost one extra residual pass per ts_step call
+   (in its last step). */
 int ts_enable_residual_norms(ts_ctx* ctx, int enable);
 
 /* Device-side log of the dt of every step (for comparing dt sequences). */

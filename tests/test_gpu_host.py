@@ -141,3 +141,77 @@ def test_restart_from_solution_file_is_seamless(tmp_path):
     tc, _, c = read_sol(c_sol, nx)
     assert tb == half and ta == tc
     np.testing.assert_array_equal(a, c)
+
+
+def test_history_and_sensors(tmp_path):
+    """history.dat (kernel.h:1646-1657, 1823-1831) every 5 iterations and the sensors of
+    src/Sensors/*.h every 7, written by the C++ host class from device reads, against the oracle
+    stepped to the same iterations (values as the reference prints them: 6 significant digits)."""
+    T = TORO["1"]
+    nx = 200
+    out = str(tmp_path / "out.bin")
+    subprocess.run([DRIVER, *map(repr, T["left"]), *map(repr, T["right"]), repr(T["x0"]), repr(T["t"]), str(nx),
+                    str(capi.RECON_LINEAR_LIM), str(capi.RIEMANN_HLLC), "3", "0", "0.0", "0", out, "-", "-",
+                    str(tmp_path)], check=True, timeout=120)
+    cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=3, bcs=[dict(type=capi.BC_NATURAL)])
+    xc = (np.arange(nx) + 0.5) / nx
+    left = xc < T["x0"]
+    U = np.zeros((1, 1, nx + 2, 5))
+    U[0, 0, 1:-1] = util.prim2cons(np.where(left, T["left"][0], T["right"][0]), np.where(left, T["left"][1], T["right"][1]),
+                                   0 * xc, 0 * xc, np.where(left, T["left"][2], T["right"][2]), 1.4)
+    o = pyoracle.Oracle(cfg)
+    o.set_state(U)
+    hist, cells = {}, {}
+    while True:
+        info, _ = o.step(1, T["t"], 0.0, log=True)
+        it = info.iteration
+        if it % 5 == 0:
+            hist[it] = (info.time, np.array(info.residual_norm))
+        if it % 7 == 0:
+            cells[it] = (info.time, capi.interior(cfg, o.get_state())[0, 0].copy())
+        if info.time >= T["t"]:
+            break
+
+    def rel(a, b):
+        return abs(a - b) <= 1e-5 * max(abs(b), 1e-300) + 1e-12
+
+    lines = open(tmp_path / "history.dat").read().strip().split("\n")
+    assert lines[0].startswith('VARIABLES = "time" "roR" "rouR" "roeR" "iter"')
+    assert len(lines) - 1 == len(hist)
+    for ln in lines[1:]:
+        t, r0, r1, r4, it = ln.split()
+        ht, hn = hist[int(it)]
+        assert rel(float(t), ht) and rel(float(r0), hn[0]) and rel(float(r1), hn[1]) and rel(float(r4), hn[4]), ln
+
+    g1 = 0.4
+    prs = lambda c: g1 * (c[:, 4] - 0.5 * c[:, 1] ** 2 / c[:, 0])  # noqa: E731
+    cl = open(tmp_path / "cell.dat").read().strip().split("\n")
+    assert cl[0].startswith('TITLE = "X=') and len(cl) - 1 == len(cells)
+    for ln in cl[1:]:
+        it, val, t = ln.split()
+        ct, st = cells[int(it)]
+        assert rel(float(val), st[nx // 2, 0]) and rel(float(t), ct), ln
+    for ln in open(tmp_path / "multi.dat").read().strip().split("\n")[1:]:
+        it, r, u, p, t = ln.split()
+        st = cells[int(it)][1]
+        assert rel(float(r), st[0, 0]) and rel(float(u), st[0, 1] / st[0, 0]) and rel(float(p), prs(st[:1])[0]), ln
+    for name, second in (("pmax1.dat", False), ("pmax2.dat", True)):
+        for ln in open(tmp_path / name).read().strip().split("\n"):
+            it, x, t = ln.split()
+            st = cells[int(it)][1]
+            p = prs(st)
+            i = int(np.argmax(p))
+            if not second:
+                want = xc[i]
+            elif i == 0:
+                want = 0.0
+            elif i == nx - 1:
+                want = 1.0
+            else:
+                # vertex of the parabola through (x_{i-1}, x_i, x_{i+1}) (MValuePosXSensor2.h:47-63)
+                h = 1.0 / nx
+                a = (p[i + 1] - 2 * p[i] + p[i - 1]) / (2 * h * h)
+                b = (p[i + 1] - p[i - 1]) / (2 * h)
+                want = xc[i] - b / (2 * a)
+            assert abs(float(x) - want) <= 1e-5, (name, ln, want)

@@ -186,6 +186,7 @@ def load(path=LIB_PATH):
     lib.ts_set_time.argtypes = [C.c_void_p, C.c_double, C.c_longlong]
     lib.ts_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, P(StepInfo)]
     lib.ts_enable_residual_norms.argtypes = [C.c_void_p, C.c_int]
+    lib.ts_get_cells.argtypes = [C.c_void_p, P(C.c_int), C.c_size_t, P(C.c_double)]
     lib.ts_enable_dt_log.argtypes = [C.c_void_p, C.c_longlong]
     lib.ts_get_dt_log.argtypes = [C.c_void_p, P(C.c_double), C.c_longlong, P(C.c_longlong)]
     lib.ts_enable_kernel_timing.argtypes = [C.c_void_p, C.c_int]
@@ -210,7 +211,7 @@ def load(path=LIB_PATH):
 
 EXPORTED = [
     "ts_create", "ts_destroy", "ts_last_error", "ts_state_size", "ts_set_state", "ts_get_state",
-    "ts_set_time", "ts_step", "ts_enable_residual_norms", "ts_enable_dt_log", "ts_get_dt_log",
+    "ts_set_time", "ts_step", "ts_get_cells", "ts_enable_residual_norms", "ts_enable_dt_log", "ts_get_dt_log",
     "ts_enable_kernel_timing", "ts_get_kernel_timing", "ts_memory_footprint",
     "ts_algorithmic_bytes_per_cell", "ts_batch_create", "ts_batch_destroy",
     "ts_batch_last_error", "ts_batch_submit", "ts_batch_run", "ts_batch_poll",
@@ -271,6 +272,13 @@ class Solver:
         info = StepInfo()
         self._check(self.lib.ts_step(self.h, max_steps, t_max, next_snapshot, C.byref(info)))
         return info
+
+    def get_cells(self, ijk):
+        """Conserved values (n, 5) of interior cells given as (n, 3) reference indices (i, j, k)."""
+        idx = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        out = np.empty((len(idx), 5))
+        self._check(self.lib.ts_get_cells(self.h, idx.ctypes.data_as(C.POINTER(C.c_int)), len(idx), _dp(out)))
+        return out
 
     def enable_residual_norms(self, enable=True):
         self._check(self.lib.ts_enable_residual_norms(self.h, int(enable)))

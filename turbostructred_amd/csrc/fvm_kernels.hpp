@@ -22,7 +22,9 @@
 //            formed on the fly, so U1 never goes to HBM.
 //   MODE_RN  residual norms sum(V |R|) per variable of UpdateSolution (kernel.h:326) over the
 //            interior, writing nothing else: per-block partial sums, and the last block adds
-//            them in block order (a deterministic sum).  Run only when the norms are asked for.
+//            them in block order (a deterministic sum).  Launched after the dt pass of every
+//            step when the norms are asked for, it runs only in the step that ends the
+//            ts_step call (the norms reported), so a call costs one extra residual pass.
 #pragma once
 
 #include "fvm_device.hpp"
@@ -529,6 +531,9 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     } else {
         if (!ctl->go) return;
+        if constexpr (MODE == MODE_RN) {
+            if (ctl->active) return;  // not the last step of this call
+        }
     }
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;

@@ -32,6 +32,9 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         }
     } else {
         if (!ctl->go) return;
+        if constexpr (MODE == MODE_RN) {
+            if (ctl->active) return;  // only in the step that ends the ts_step call
+        }
     }
     const GridArgs& g = a.g;
     const int lane = threadIdx.x & 63;

@@ -52,6 +52,9 @@ struct ts_ctx {
     bool timing = false;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};  // per kind: dt, stages 1-3, (unused), residual norms
     long long t_n[6] = {0, 0, 0, 0, 0, 0};
+    unsigned* d_cells = nullptr;  // ts_get_cells: cell indices and gathered values
+    double* d_cell_vals = nullptr;
+    size_t cells_cap = 0;
     bool norms = false;         // residual-norm pass after the first kernel of every step
     double* rn_part = nullptr;  // its per-block partial sums
     double h_rnorm[5] = {0, 0, 0, 0, 0};
@@ -82,4 +85,5 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg);
 void destroy_device_ctx(ts_ctx* c);
 void aos_to_soa(const ts_ctx* c, const double* aos, double* soa);
 void soa_to_aos(const ts_ctx* c, const double* soa, double* aos);
+int get_cells(ts_ctx* c, const int* ijk, size_t n, double* out);
 int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap);

@@ -235,6 +235,12 @@ int ts_get_state(ts_ctx* c, double* aos, size_t n) {
     return TS_OK;
 }
 
+int ts_get_cells(ts_ctx* c, const int* ijk, size_t n, double* out) {
+    if (!c || (n && (!ijk || !out))) return TS_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return get_cells(c, ijk, n, out);
+}
+
 int ts_set_time(ts_ctx* c, double time, long long iteration) {
     if (!c) return TS_ERR_INVALID;
     c->h_time = time;

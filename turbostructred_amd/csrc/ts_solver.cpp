@@ -199,6 +199,8 @@ void destroy_device_ctx(ts_ctx* c) {
     c->dbufs.clear();
     if (c->dt_log) hipFree(c->dt_log);
     if (c->rn_part) hipFree(c->rn_part);
+    if (c->d_cells) hipFree(c->d_cells);
+    if (c->d_cell_vals) hipFree(c->d_cell_vals);
     if (c->hctl) hipHostFree(c->hctl);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -388,6 +390,52 @@ int issue_graph(ts_ctx* c, int start_buf, int count) {
 }
 
 }  // namespace
+
+namespace {
+__global__ void __launch_bounds__(256) gather_cells(const double* __restrict__ u, size_t plane,
+                                                    const unsigned* __restrict__ idx, int n,
+                                                    double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 5) return;
+    const int c = t / 5, v = t - c * 5;
+    out[t] = u[(size_t)v * plane + idx[c]];
+}
+}  // namespace
+
+int get_cells(ts_ctx* c, const int* ijk, size_t n, double* out) {
+    if (n == 0) return TS_OK;
+    std::vector<unsigned> idx(n);
+    const int lo[3] = {1, c->nd > 1 ? 1 : 0, c->nd > 2 ? 1 : 0};
+    for (size_t q = 0; q < n; q++) {
+        int p[3];
+        for (int a = 0; a < 3; a++) {
+            p[a] = ijk[3 * q + a] - lo[a];
+            if (p[a] < 0 || p[a] >= c->n[a]) {
+                c->err = "cell index outside the interior";
+                return TS_ERR_INVALID;
+            }
+        }
+        idx[q] = (unsigned)(((size_t)p[2] * c->n[1] + p[1]) * c->n[0] + p[0]);
+    }
+    if (n > c->cells_cap) {
+        if (c->d_cells) CHK(hipFree(c->d_cells));
+        if (c->d_cell_vals) CHK(hipFree(c->d_cell_vals));
+        c->d_cells = nullptr;
+        c->d_cell_vals = nullptr;
+        c->cells_cap = 0;
+        CHK(hipMalloc((void**)&c->d_cells, n * sizeof(unsigned)));
+        CHK(hipMalloc((void**)&c->d_cell_vals, n * 5 * sizeof(double)));
+        c->cells_cap = n;
+    }
+    CHK(hipMemcpyAsync(c->d_cells, idx.data(), n * sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+    const int work = (int)(n * 5);
+    hipLaunchKernelGGL(gather_cells, dim3((work + 255) / 256), dim3(256), 0, c->stream, (const double*)c->dU[c->cur],
+                       (size_t)c->ncell, c->d_cells, (int)n, c->d_cell_vals);
+    CHK(hipGetLastError());
+    CHK(hipMemcpyAsync(out, c->d_cell_vals, n * 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    return TS_OK;
+}
 
 int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap) {
     StepCtl* h = c->hctl;

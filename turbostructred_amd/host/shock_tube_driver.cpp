@@ -4,11 +4,14 @@
 // SetInitialConditions, Run.
 //
 // usage: shock_tube_driver rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress
-//                          save_time residual_norms out.bin [save.sol load.sol]
+//                          save_time residual_norms out.bin [save.sol load.sol [probe_dir]]
 //   recon / riemann: the reference's enum values; rk: 1 or 3; compress: 1 = the two-block
 //   geometric grid of RunSODTestRoe1D (q = 1/1.05 then 1.05, node at 0.5 L), 0 = uniform;
 //   save_time > 0 clamps dt onto its multiples (SaveSolutionTime); save.sol / load.sol ("-" for
-//   none): Kernel::SaveSolution after the run / Kernel::LoadSolution before it (the restart path).
+//   none): Kernel::SaveSolution after the run / Kernel::LoadSolution before it (the restart path);
+//   probe_dir: history.dat every 5 iterations and, every 7, a density sensor at the middle cell,
+//   a (rho, u, p) sensor at the first cell and the position of the pressure maximum (first and
+//   second order), written there.
 // out.bin: int32 nXAll, int32 snapshots, int32 iteration, pad; double time, dt, residual[5];
 //          CoordinateX[nXAll], hx[nXAll], values[nXAll*5].
 #include <cstdio>
@@ -22,14 +25,15 @@
 using namespace tsh;
 
 int main(int argc, char** argv) {
-    if (argc != 17 && argc != 19) {
+    if (argc != 17 && argc != 19 && argc != 20) {
         std::fprintf(stderr,
                      "usage: %s rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress save_time norms out.bin"
-                     " [save.sol load.sol]\n",
+                     " [save.sol load.sol [probe_dir]]\n",
                      argv[0]);
         return 2;
     }
-    const std::string save_sol = argc == 19 ? argv[17] : "-", load_sol = argc == 19 ? argv[18] : "-";
+    const std::string save_sol = argc >= 19 ? argv[17] : "-", load_sol = argc >= 19 ? argv[18] : "-";
+    const std::string probe_dir = argc == 20 ? argv[19] : "-";
     const double roL = std::atof(argv[1]), uL = std::atof(argv[2]), PL = std::atof(argv[3]);
     const double roR = std::atof(argv[4]), uR = std::atof(argv[5]), PR = std::atof(argv[6]);
     const double x0 = std::atof(argv[7]), t_end = std::atof(argv[8]);
@@ -79,6 +83,27 @@ int main(int argc, char** argv) {
             return std::vector<double>{ro, ro * u, 0.0, 0.0, p / g1 + 0.5 * ro * u * u};
         });
         if (load_sol != "-") kernel->LoadSolution(load_sol);
+        if (probe_dir != "-") {
+            kernel->SaveHistoryFileIters = 5;
+            kernel->OutputDirectory = probe_dir;
+            kernel->isSensorEnable = true;
+            kernel->SaveSensorRecordIters = 7;
+            auto rho = [](const std::valarray<double>& U) { return U[0]; };
+            auto vel = [](const std::valarray<double>& U) { return U[1] / U[0]; };
+            auto prs = [g1](const std::valarray<double>& U) { return g1 * (U[4] - 0.5 * U[1] * U[1] / U[0]); };
+            auto c1 = std::make_unique<CellSensor>(probe_dir + "/cell.dat", rho);
+            c1->SetSensor(*kernel, nx / 2 + 1, 0, 0);
+            auto c2 = std::make_unique<CellSensor>(probe_dir + "/multi.dat", std::vector<sen_func>{rho, vel, prs});
+            c2->SetSensor(*kernel, 1, 0, 0);
+            auto m1 = std::make_unique<MValuePosXSensor>(probe_dir + "/pmax1.dat", prs, false);
+            m1->SetSensor(*kernel, 0, 0);
+            auto m2 = std::make_unique<MValuePosXSensor>(probe_dir + "/pmax2.dat", prs, true);
+            m2->SetSensor(*kernel, 0, 0);
+            kernel->Sensors.push_back(std::move(c1));
+            kernel->Sensors.push_back(std::move(c2));
+            kernel->Sensors.push_back(std::move(m1));
+            kernel->Sensors.push_back(std::move(m2));
+        }
         int snapshots = 0;
         kernel->OnSolutionSnapshot = [&](HipExplicitRungeKuttaFVM&) { snapshots++; };
         kernel->Run();

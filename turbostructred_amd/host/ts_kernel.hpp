@@ -15,8 +15,12 @@
 
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
+#include <fstream>
 #include <functional>
+#include <iostream>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -101,6 +105,11 @@ struct KernelConfiguration {
     double SaveSolutionTime{0};
     double SaveSliceTime{0};
     int SaveSolutionIters{0};
+    int ResidualOutputIters{0};
+    int ChangeHistoryFileIters{5000000};
+    // history.dat every SaveHistoryFileIters iterations; the reference's default is 10, here it
+    // is 0 (no file) so that a library run writes nothing unless asked
+    int SaveHistoryFileIters{0};
     std::map<int, BoundaryConditionConfiguration> MyConditions;
     DomainBCinfo xLeftBoundary, xRightBoundary, yLeftBoundary, yRightBoundary, zLeftBoundary, zRightBoundary;
     int Device{0};  // HIP device ordinal (extension)
@@ -165,6 +174,23 @@ inline void BuildAxis(std::vector<BlockNode> blocks, int n, double L, int ghost,
     }
 }
 
+class HipExplicitRungeKuttaFVM;
+
+// src/Sensors/Sensor.h: a probe processed every SaveSensorRecordIters iterations.  Here a sensor
+// reads only the cells it needs from the device (ts_get_cells) instead of the whole state.
+class Sensor {
+   public:
+    int iteration{0};
+    double timer{0};
+    std::string filename;
+    explicit Sensor(std::string fname) : filename(std::move(fname)) {}
+    virtual ~Sensor() = default;
+    virtual void Process(HipExplicitRungeKuttaFVM& kernel) = 0;
+    void UpdateIteration(int it) { iteration = it; }
+    void UpdateTimer(double t) { timer = t; }
+};
+using sen_func = std::function<double(const std::valarray<double>&)>;
+
 // The explicit finite-volume method on one GPU: Kernel + ExplicitRungeKuttaFVM<R> with the
 // reconstruction chosen at run time (the reference picks it as a template argument in
 // CreateKernel, ExplicitRungeKuttaFVM.h:1495-1515).
@@ -182,6 +208,14 @@ class HipExplicitRungeKuttaFVM {
     double SaveSolutionTime{0};
     double SaveSliceTime{0};
     bool ComputeResidualNorms{false};
+    int ResidualOutputIters{0};
+    int SaveHistoryFileIters{0};
+    int ChangeHistoryFileIters{5000000};
+    int NextHistoryFileIters{5000000};
+    std::string OutputDirectory{"."};  // where history files go
+    bool isSensorEnable{false};
+    int SaveSensorRecordIters{1};
+    std::vector<std::unique_ptr<Sensor>> Sensors;
     // called at every solution / slice snapshot time with `values` synchronised
     std::function<void(HipExplicitRungeKuttaFVM&)> OnSolutionSnapshot, OnSliceSnapshot;
 
@@ -289,6 +323,10 @@ class HipExplicitRungeKuttaFVM {
 
         MaxIteration = config.MaxIteration;
         MaxTime = config.MaxTime;
+        ResidualOutputIters = config.ResidualOutputIters;
+        SaveHistoryFileIters = config.SaveHistoryFileIters;
+        ChangeHistoryFileIters = config.ChangeHistoryFileIters;
+        NextHistoryFileIters = ChangeHistoryFileIters;  // kernel.h:252-254
         SaveSolutionTime = config.SaveSolutionTime;
         SaveSliceTime = config.SaveSliceTime;
         stepInfo = StepInfo();
@@ -334,19 +372,75 @@ class HipExplicitRungeKuttaFVM {
     // Time >= MaxTime, with dt clamped onto the solution / slice snapshot times (ERK:1433-1434)
     // and the snapshot hooks called there.  Steps between snapshots run on the device without
     // a host round trip.
+    //
+    // Per iteration the reference (kernel.h:1701-1850) prints residuals every
+    // ResidualOutputIters, processes the sensors every SaveSensorRecordIters, handles the
+    // solution/slice snapshots, and appends a history.dat line every SaveHistoryFileIters
+    // (starting history<n>.dat past each ChangeHistoryFileIters).  Here the device runs
+    // uninterrupted up to the next such event; the residual norms of the step that ends a
+    // device run are computed on the device when an event needs them.
     void Run() {
-        EnableNorms();
+        const bool hist = SaveHistoryFileIters > 0;
+        std::ofstream hofs;
+        if (hist) OpenHistory(hofs, OutputDirectory + "/history.dat");
+        const bool norms = ComputeResidualNorms || hist || ResidualOutputIters > 0;
+        SetNorms(norms);
+        const bool sensing = isSensorEnable && !Sensors.empty() && SaveSensorRecordIters > 0;
         while (true) {
             // the reference steps first and tests afterwards, so a run always takes a step, and
             // one started past MaxIteration only ends on time
-            const int left = stepInfo.Iteration < MaxIteration ? MaxIteration - stepInfo.Iteration : 1 << 30;
+            const int it = stepInfo.Iteration;
+            long long left = it < MaxIteration ? MaxIteration - it : 1 << 30;
+            auto upto = [&](int every) {
+                if (every > 0) left = std::min<long long>(left, every - ((it % every) + every) % every);
+            };
+            if (hist) upto(SaveHistoryFileIters);
+            if (sensing) upto(SaveSensorRecordIters);
+            if (ResidualOutputIters > 0) upto(ResidualOutputIters);
+            if (hist && NextHistoryFileIters >= it) left = std::min<long long>(left, NextHistoryFileIters + 1 - it);
+            if (left < 1) left = 1;
             ts_step_info info{};
-            check(ts_step(ctx_, left, MaxTime, NextSnapshot(), &info));
+            check(ts_step(ctx_, (int)left, MaxTime, NextSnapshot(), &info));
             Absorb(info);
             if (info.steps_done == 0) break;
+            const int now = stepInfo.Iteration;
+            if (ResidualOutputIters > 0 && now % ResidualOutputIters == 0)
+                std::cout << "Iteration = " << now << "; Total time = " << stepInfo.Time
+                          << "; Time step = " << stepInfo.TimeStep << "; RMSrou = " << stepInfo.Residual[1] << std::endl;
+            if (sensing && now % SaveSensorRecordIters == 0)
+                for (auto& sn : Sensors) {
+                    sn->UpdateIteration(now);
+                    sn->UpdateTimer(stepInfo.Time);
+                    sn->Process(*this);
+                }
             Snapshots();
+            if (hist) {
+                if (now > NextHistoryFileIters) {
+                    const int n_his = now / ChangeHistoryFileIters;
+                    NextHistoryFileIters += ChangeHistoryFileIters;
+                    hofs.close();
+                    OpenHistory(hofs, OutputDirectory + "/history" + std::to_string(n_his) + ".dat");
+                }
+                if (now % SaveHistoryFileIters == 0) {
+                    hofs << stepInfo.Time << " " << stepInfo.Residual[0] << " " << stepInfo.Residual[1] << " ";
+                    if (nDims > 1) hofs << stepInfo.Residual[2] << " ";
+                    if (nDims > 2) hofs << stepInfo.Residual[3] << " ";
+                    hofs << stepInfo.Residual[4] << " " << now << std::endl;
+                }
+            }
             if (stepInfo.Iteration == MaxIteration || stepInfo.Time >= MaxTime) break;
         }
+    }
+
+    // values of interior cells (reference indices i, j, k with the ghost offset) from the device
+    std::vector<std::valarray<double>> GetCells(const std::vector<int>& ijk) {
+        const size_t n = ijk.size() / 3;
+        std::vector<double> raw(n * 5);
+        check(ts_get_cells(ctx_, ijk.data(), n, raw.data()));
+        std::vector<std::valarray<double>> out(n, std::valarray<double>(5));
+        for (size_t c = 0; c < n; c++)
+            for (int v = 0; v < 5; v++) out[c][v] = raw[c * 5 + v];
+        return out;
     }
 
     // Kernel::SaveSolution (kernel.h:992-1029), as one rank writes it: the time (double), the
@@ -404,11 +498,21 @@ class HipExplicitRungeKuttaFVM {
     void check(int rc) {
         if (rc != TS_OK) throw std::runtime_error("libts_fvm: " + std::to_string(rc) + ": " + ts_last_error(ctx_));
     }
-    void EnableNorms() {
-        if (ComputeResidualNorms != norms_on_) {
-            check(ts_enable_residual_norms(ctx_, ComputeResidualNorms ? 1 : 0));
-            norms_on_ = ComputeResidualNorms;
+    void EnableNorms() { SetNorms(ComputeResidualNorms); }
+    void SetNorms(bool on) {
+        if (on != norms_on_) {
+            check(ts_enable_residual_norms(ctx_, on ? 1 : 0));
+            norms_on_ = on;
         }
+    }
+    // history file header (kernel.h:1646-1657, 1808-1819)
+    void OpenHistory(std::ofstream& ofs, const std::string& path) {
+        ofs.open(path, std::ios_base::out);
+        if (!ofs) throw std::runtime_error("cannot open " + path);
+        ofs << "VARIABLES = \"time\" \"roR\" \"rouR\" ";
+        if (nDims > 1) ofs << "\"rovR\" ";
+        if (nDims > 2) ofs << "\"rowR\" ";
+        ofs << "\"roeR\" \"iter\" " << std::endl;
     }
     double NextSnapshot() const {
         double t = 0.0;
@@ -444,6 +548,94 @@ class HipExplicitRungeKuttaFVM {
         }
     }
 };
+
+// src/Sensors/CellSensor.h and CellMultiSensor.h: function(s) of the state of one cell, one line
+// "iteration value(s) time" per record; the file starts with the cell centre (Sensor.h:55-63)
+class CellSensor : public Sensor {
+   public:
+    CellSensor(std::string fname, sen_func f) : Sensor(std::move(fname)), fs_{std::move(f)} {}
+    CellSensor(std::string fname, std::vector<sen_func> fs) : Sensor(std::move(fname)), fs_(std::move(fs)) {
+        multi_ = true;
+    }
+    void SetSensor(const HipExplicitRungeKuttaFVM& k, int i, int j, int kk);
+    void Process(HipExplicitRungeKuttaFVM& k) override {
+        if (!active_) return;
+        std::valarray<double> U = k.GetCells(ijk_)[0];
+        std::ofstream ofs(filename, std::ios_base::app);
+        ofs << iteration << ' ';
+        if (multi_) {
+            for (auto& f : fs_) ofs << f(U) << ' ';
+        } else {
+            ofs << fs_[0](U) << ' ';
+        }
+        ofs << timer << std::endl;
+    }
+
+   private:
+    std::vector<sen_func> fs_;
+    std::vector<int> ijk_;
+    bool active_{false}, multi_{false};
+};
+
+// src/Sensors/MValuePosXSensor.h / MValuePosXSensor2.h: position of the maximum of a function
+// along the x-line (j, k); `second_order` places it at the vertex of the parabola through the
+// maximum cell and its neighbours on the (non-uniform) grid, the end cells at their outer faces
+class MValuePosXSensor : public Sensor {
+   public:
+    MValuePosXSensor(std::string fname, sen_func f, bool second_order = false)
+        : Sensor(std::move(fname)), f_(std::move(f)), second_(second_order) {}
+    void SetSensor(const HipExplicitRungeKuttaFVM& k, int j_slice, int k_slice);
+    void Process(HipExplicitRungeKuttaFVM& k) override;
+
+   private:
+    sen_func f_;
+    bool second_;
+    std::vector<int> ijk_;
+};
+
+inline void CellSensor::SetSensor(const HipExplicitRungeKuttaFVM& k, int i, int j, int kk) {
+    ijk_ = {i, j, kk};
+    active_ = true;
+    std::ofstream ofs(filename);
+    ofs << "TITLE = \"X=" << k.CoordinateX[i] << ", Y=" << (k.nDims > 1 ? k.CoordinateY[j] : 0.0)
+        << ", Z=" << (k.nDims > 2 ? k.CoordinateZ[kk] : 0.0) << '"' << std::endl;
+}
+
+inline void MValuePosXSensor::SetSensor(const HipExplicitRungeKuttaFVM& k, int j_slice, int k_slice) {
+    // j_slice, k_slice: interior indices without the ghost offset (MValuePosXSensor.h:22-30)
+    ijk_.clear();
+    const int J = k.nDims > 1 ? j_slice + 1 : 0, K = k.nDims > 2 ? k_slice + 1 : 0;
+    for (int i = 1; i <= k.nX; i++) ijk_.insert(ijk_.end(), {i, J, K});
+}
+
+inline void MValuePosXSensor::Process(HipExplicitRungeKuttaFVM& k) {
+    if (ijk_.empty()) return;
+    const std::vector<std::valarray<double>> row = k.GetCells(ijk_);
+    double uMax = -std::numeric_limits<double>::max();
+    int iMax = 0;
+    std::vector<double> u(row.size());
+    for (size_t i = 0; i < row.size(); i++) {
+        u[i] = f_(row[i]);
+        if (u[i] > uMax) uMax = u[i], iMax = (int)i;
+    }
+    const int I = 1 + iMax;  // index with the ghost offset
+    double xmax = k.CoordinateX[I];
+    if (second_) {
+        if (iMax == 0) {
+            xmax = k.CoordinateX[I] - 0.5 * k.hx[I];
+        } else if (iMax == k.nX - 1) {
+            xmax = k.CoordinateX[I] + 0.5 * k.hx[I];
+        } else {
+            const double xi = k.CoordinateX[I], dxp = k.CoordinateX[I + 1] - xi, dxm = xi - k.CoordinateX[I - 1];
+            const double v2 = uMax - u[iMax - 1], v3 = u[iMax + 1] - u[iMax - 1];
+            const double adxm = v3 * dxm / ((dxm + dxp) * dxp) - v2 / dxp;
+            const double bda = dxm + v2 / adxm;
+            xmax = xi - 0.5 * bda;
+        }
+    }
+    std::ofstream ofs(filename, std::ios_base::app);
+    ofs << iteration << ' ' << xmax << ' ' << timer << std::endl;
+}
 
 // CreateKernel (ExplicitRungeKuttaFVM.h:1495-1515)
 inline std::unique_ptr<HipExplicitRungeKuttaFVM> CreateKernel(KernelConfiguration&) {

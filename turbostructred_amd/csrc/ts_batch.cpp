@@ -402,7 +402,7 @@ int rebuild_tables(ts_batch* b, Family& f) {
         realloc = true;
     }
     if (realloc && f.graph) {
-        hipGraphExecDestroy(f.graph);
+        (void)hipGraphExecDestroy(f.graph);
         f.graph = nullptr;
     }
     // descriptors: one per (kernel, problem), kernel k's block starting at k * cap_desc so that
@@ -450,7 +450,7 @@ int issue_steps(ts_batch* b, Family& f, int n) {
         for (int s = 0; s < n; s++) issue_step(b, f);
         BCHK(hipStreamEndCapture(b->stream, &gr));
         hipError_t e = hipGraphInstantiate(&f.graph, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
+        (void)hipGraphDestroy(gr);
         if (e != hipSuccess) {
             f.graph = nullptr;
             return fail(b, TS_ERR_HIP, tsr::hip_msg(e, "hipGraphInstantiate"));
@@ -511,7 +511,7 @@ int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out) {
         (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_arena, b->bc.hbm_budget_bytes)) != hipSuccess) {
         fprintf(stderr, "ts_batch_create: %s\n", hipGetErrorString(e));
-        if (b->stream) hipStreamDestroy(b->stream);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return e == hipErrorOutOfMemory ? TS_ERR_NOMEM : TS_ERR_HIP;
     }
@@ -525,20 +525,20 @@ int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out) {
 int ts_batch_destroy(ts_batch* b) {
     if (!b) return TS_ERR_INVALID;
     ts_batch_stop_server(b);
-    hipSetDevice(b->bc.device);
-    hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->bc.device);
+    (void)hipStreamSynchronize(b->stream);
     for (Family& f : b->families) {
-        if (f.graph) hipGraphExecDestroy(f.graph);
-        if (f.d_desc) hipFree(f.d_desc);
-        if (f.d_table) hipFree(f.d_table);
-        if (f.d_ntiles) hipFree(f.d_ntiles);
+        if (f.graph) (void)hipGraphExecDestroy(f.graph);
+        if (f.d_desc) (void)hipFree(f.d_desc);
+        if (f.d_table) (void)hipFree(f.d_table);
+        if (f.d_ntiles) (void)hipFree(f.d_ntiles);
     }
-    if (b->h_ctl) hipHostFree(b->h_ctl);
-    if (b->h_ptr) hipHostFree(b->h_ptr);
-    if (b->d_ctl) hipFree(b->d_ctl);
-    if (b->d_ptr) hipFree(b->d_ptr);
-    if (b->d_arena) hipFree(b->d_arena);
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+    if (b->h_ptr) (void)hipHostFree(b->h_ptr);
+    if (b->d_ctl) (void)hipFree(b->d_ctl);
+    if (b->d_ptr) (void)hipFree(b->d_ptr);
+    if (b->d_arena) (void)hipFree(b->d_arena);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TS_OK;
 }

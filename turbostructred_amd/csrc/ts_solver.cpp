@@ -192,18 +192,18 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
 }
 
 void destroy_device_ctx(ts_ctx* c) {
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; i++)
-        if (c->graph_exec[i]) hipGraphExecDestroy(c->graph_exec[i]);
-    for (double* p : c->dbufs) hipFree(p);
+        if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+    for (double* p : c->dbufs) (void)hipFree(p);
     c->dbufs.clear();
-    if (c->dt_log) hipFree(c->dt_log);
-    if (c->rn_part) hipFree(c->rn_part);
-    if (c->d_cells) hipFree(c->d_cells);
-    if (c->d_cell_vals) hipFree(c->d_cell_vals);
-    if (c->hctl) hipHostFree(c->hctl);
-    for (hipEvent_t e : c->events) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->dt_log) (void)hipFree(c->dt_log);
+    if (c->rn_part) (void)hipFree(c->rn_part);
+    if (c->d_cells) (void)hipFree(c->d_cells);
+    if (c->d_cell_vals) (void)hipFree(c->d_cell_vals);
+    if (c->hctl) (void)hipHostFree(c->hctl);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 void aos_to_soa(const ts_ctx* c, const double* aos, double* soa) {
@@ -377,7 +377,7 @@ int issue_graph(ts_ctx* c, int start_buf, int count) {
         }
         CHK(hipStreamEndCapture(c->stream, &graph));
         hipError_t e = hipGraphInstantiate(&c->graph_exec[gi], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) {
             c->err = tsr::hip_msg(e, "hipGraphInstantiate");
             c->graph_exec[gi] = nullptr;

@@ -36,7 +36,9 @@ def axes():
         src = os.path.join(d, "probe.cpp")
         exe = os.path.join(d, "probe")
         open(src, "w").write(PROBE)
-        subprocess.run(["g++", "-std=c++17", "-O1", "-I", HOST, src, "-o", exe], check=True)
+        pkg = os.path.join(ROOT, "turbostructred_amd")
+        subprocess.run(["g++", "-std=c++17", "-O1", "-I", HOST, src, "-o", exe, "-L", pkg, "-lts_fvm",
+                        f"-Wl,-rpath,{pkg}"], check=True)
         out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
     rows = {"U": [], "C": []}
     for line in out.split("\n"):

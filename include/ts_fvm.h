@@ -181,7 +181,6 @@ typedef struct ts_batch_config {
     size_t hbm_budget_bytes; /* admission limit on the device memory of resident problems */
     int max_active;          /* maximum number of problems advanced together */
     int use_graphs;          /* capture the per-step launches of a fixed batch in a hipGraph */
-    int autotune;            /* pick tile shapes with the built-in autotuner */
 } ts_batch_config;
 
 int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out);
@@ -208,8 +207,9 @@ int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state,
 int ts_batch_start_server(ts_batch* b);
 int ts_batch_stop_server(ts_batch* b);
 
-/* Autotuner: time the candidate tile shapes for cfg's kernel family at cfg's size and store
-   the winner in the process-wide tuning table (also used by later ts_create calls). */
+/* Autotuner: time the candidate tile heights for cfg's kernel family at cfg's size (the
+   occupancy-model choice, its neighbours and 4..64 rows) and store the winner in the
+   process-wide tuning table used by later ts_create calls.  Results never depend on it. */
 int ts_autotune(const ts_config* cfg, int* chosen_rows_per_tile, double* best_ms);
 
 #ifdef __cplusplus

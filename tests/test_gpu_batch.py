@@ -178,3 +178,37 @@ def test_batched_stretched_grid_equals_single_solve():
     st, info = out[t]
     assert info.steps_done == 60 and info.time == rinfo.time
     np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, ref))
+
+
+def test_server_thread_equals_driven_batch():
+    """ts_batch_start_server: a background thread advances the batch; solves submitted while it
+    runs finish with the same bits as the same solves driven by ts_batch_run."""
+    import time
+    probs = [quad(n, m, s, precision=capi.FP32_RESIDUAL) for s, (n, m) in enumerate([(64, 64), (130, 70), (97, 150)])]
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    tk = [b.submit(cfg, U, 60) for cfg, U in probs]
+    ref = drain(b, list(tk))
+    b.close()
+    s = capi.Batch(hbm_budget_bytes=1 << 28)
+    s.start_server()
+    try:
+        tk2 = []
+        for cfg, U in probs:
+            tk2.append(s.submit(cfg, U, 60))
+            time.sleep(0.002)
+        got = {}
+        deadline = time.time() + 60
+        while len(got) < len(tk2) and time.time() < deadline:
+            for t in tk2:
+                if t not in got:
+                    d, st, info = s.poll(t)
+                    if d:
+                        got[t] = (st, info)
+            time.sleep(0.001)
+        assert len(got) == len(tk2), "server did not finish the solves"
+    finally:
+        s.stop_server()
+        s.close()
+    for t1, t2 in zip(tk, tk2):
+        assert got[t2][1].steps_done == ref[t1][1].steps_done == 60
+        np.testing.assert_array_equal(got[t2][0], ref[t1][0])

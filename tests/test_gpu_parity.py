@@ -295,3 +295,23 @@ def test_residual_norms(nd, riemann, recon, stages):
     assert all(x == 0.0 for x in gi.residual_norm)
     err = util.normwise_rel(capi.interior(cfg, gpu.get_state()), capi.interior(cfg, ref.get_state()))
     assert np.all(err <= TOL), err
+
+
+def test_autotune_changes_nothing_but_the_tiling():
+    """ts_autotune times the candidate tile heights and stores the winner for later contexts; a
+    solve after it is bitwise the solve before it (a cell's arithmetic does not depend on the
+    tiling)."""
+    cfg = capi.make_config(ndims=2, nx=300, ny=260, cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3, bcs=[dict(type=capi.BC_NATURAL)])
+    U = util.quadrants(cfg)
+
+    def run():
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        s.step(30)
+        return s.get_state()
+
+    before = run()
+    ty, ms = capi.autotune(cfg)
+    assert 1 <= ty <= 260 and ms > 0
+    np.testing.assert_array_equal(run(), before)

@@ -87,7 +87,6 @@ class BatchConfig(C.Structure):
         ("hbm_budget_bytes", C.c_size_t),
         ("max_active", C.c_int),
         ("use_graphs", C.c_int),
-        ("autotune", C.c_int),
     ]
 
 
@@ -315,6 +314,16 @@ class Solver:
         return b.value
 
 
+def autotune(cfg):
+    """Time the candidate tile heights for cfg (ts_autotune): returns (rows per tile, ms/step)."""
+    lib = load()
+    ty, ms = C.c_int(), C.c_double()
+    rc = lib.ts_autotune(C.byref(cfg), C.byref(ty), C.byref(ms))
+    if rc != TS_OK:
+        raise TSError(rc, "ts_autotune failed")
+    return ty.value, ms.value
+
+
 class Batch:
     """Owner of one ts_batch: many independent solves advanced together (continuous batching)."""
 
@@ -325,7 +334,6 @@ class Batch:
         bc.hbm_budget_bytes = hbm_budget_bytes
         bc.max_active = max_active
         bc.use_graphs = int(use_graphs)
-        bc.autotune = 0
         h = C.c_void_p()
         rc = self.lib.ts_batch_create(C.byref(bc), C.byref(h))
         if rc != TS_OK:

@@ -5,6 +5,7 @@
 // the same whatever the tiling, so the table may be keyed by problem size alone.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -72,7 +73,24 @@ extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
         if (best_ms) *best_ms = 0;
         return cfg ? TS_OK : TS_ERR_INVALID;
     }
-    const int cands[] = {4, 8, 16, 32, 64};
+    // the occupancy model's choice (with no table entry) and its neighbours, then fixed heights
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_table.erase(key_of(*cfg));
+    }
+    std::vector<int> cands;
+    {
+        ts_ctx* c = nullptr;
+        if (ts_create(cfg, &c) == TS_OK) {
+            cands = {c->ty, c->ty > 5 ? c->ty - c->ty / 4 : 4, c->ty + c->ty / 4 + 1};
+            ts_destroy(c);
+        }
+    }
+    for (int ty : {4, 8, 16, 32, 64}) cands.push_back(ty);
+    const int ymax = cfg->ny;
+    std::vector<int> uniq;
+    for (int ty : cands)
+        if (ty >= 1 && ty <= std::max(ymax, 4) && std::find(uniq.begin(), uniq.end(), ty) == uniq.end()) uniq.push_back(ty);
     size_t nst = ts_state_size_cfg(cfg);
     std::vector<double> state(nst, 0.0);
     // a quiescent uniform state: arithmetic cost does not depend on the data
@@ -82,7 +100,7 @@ extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
     }
     int best = -1;
     double bms = 1e300;
-    for (int ty : cands) {
+    for (int ty : uniq) {
         {
             std::lock_guard<std::mutex> lk(g_mu);
             g_table[key_of(*cfg)] = ty;

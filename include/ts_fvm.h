@@ -106,6 +106,13 @@ typedef struct ts_config {
     int side_marker[6];
     const uint8_t* face_marker[6];
     int device;          /* HIP device ordinal */
+    /* viscous terms and body forces (KernelConfiguration.h:41-51, 86-88).  FP64 only. */
+    int viscous;         /* IsViscousFlow: viscous fluxes from cell velocity gradients (ERK:79-874) */
+    double viscosity;    /* Viscosity; also enters the viscous spectral-radius term (ERK:1239-1248) */
+    int external_force;  /* IsExternalForceRequared: residual += V Sigma, V Sigma.u (kernel.h:941-965) */
+    double sigma[3];     /* Sigma: potential force, e.g. a pressure gradient */
+    int external_accel;  /* IsUnifromAccelerationRequared: residual += rho V g, rho V g.u (kernel.h:966-989) */
+    double accel[3];     /* UniformAcceleration */
 } ts_config;
 
 /* StepInfo (kernel.h:56) plus the per-variable residual norms of UpdateSolution (kernel.h:326). */

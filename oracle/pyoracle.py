@@ -41,6 +41,8 @@ def load():
     lib.tso_exact_sample.argtypes = [C.c_double] * 8 + [D]
     lib.tso_reconstruct.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, D, D]
     lib.tso_bc_image.argtypes = [P(BCConfig), C.c_double, D, D, D]
+    lib.tso_velocity_gradients.argtypes = [C.c_void_p, D, C.c_size_t]
+    lib.tso_lsq_gradient.argtypes = [C.c_int, D, C.c_double, D, D, C.c_int, D]
     _lib = lib
     return lib
 
@@ -104,6 +106,12 @@ class Oracle:
         assert self.lib.tso_last_residual(self.h, _dp(r), _dp(sr), n) == 0
         return r, sr
 
+    def velocity_gradients(self):
+        n = self.state_size() // 5
+        g = np.empty(n * 9)
+        assert self.lib.tso_velocity_gradients(self.h, _dp(g), g.size) == 0
+        return g.reshape(n, 3, 3)
+
 
 def flux(riemann, gamma, UL, UR, n, eps=0.0):
     lib = load()
@@ -146,3 +154,11 @@ def bc_image(bc, gamma, U, n):
     U, n = _arr(U), _arr(n)
     lib.tso_bc_image(C.byref(bc), gamma, _dp(U), _dp(n), _dp(out))
     return out
+
+
+def lsq_gradient(ndims, point, value, points, values):
+    lib = load()
+    point, pts, vals = _arr(point), _arr(points).reshape(-1, 3), _arr(values)
+    g = np.empty(3)
+    lib.tso_lsq_gradient(ndims, _dp(point), float(value), _dp(pts), _dp(vals), len(vals), _dp(g))
+    return g

@@ -415,6 +415,45 @@ void bc_image(const ts_bc_config& bc, double g, const double* U, const double* n
     }
 }
 
+/* Least-squares gradient at `point` from the value there and at m stencil points
+   (GradientComputer.h:19-53): row i is point - p_i (ndims columns), right-hand side
+   value - value_i.  The reference solves it with Eigen's JacobiSVD; for the full-rank stencils
+   used here that is the unique least-squares solution, found here through the normal equations
+   and a Cholesky factorisation. */
+void lsq_gradient(int nd, const double* point, double value, const double (*pts)[3], const double* vals,
+                  int m, double* grad) {
+    double M[3][3] = {{0}}, b[3] = {0, 0, 0};
+    for (int i = 0; i < m; i++) {
+        double r[3];
+        for (int a = 0; a < nd; a++) r[a] = point[a] - pts[i][a];
+        double rhs = value - vals[i];
+        for (int a = 0; a < nd; a++) {
+            for (int c = 0; c < nd; c++) M[a][c] += r[a] * r[c];
+            b[a] += r[a] * rhs;
+        }
+    }
+    double L[3][3] = {{0}};
+    for (int a = 0; a < nd; a++) {
+        for (int c = 0; c <= a; c++) {
+            double s = M[a][c];
+            for (int q = 0; q < c; q++) s -= L[a][q] * L[c][q];
+            L[a][c] = (a == c) ? std::sqrt(s) : s / L[c][c];
+        }
+    }
+    double y[3];
+    for (int a = 0; a < nd; a++) {
+        double s = b[a];
+        for (int q = 0; q < a; q++) s -= L[a][q] * y[q];
+        y[a] = s / L[a][a];
+    }
+    for (int a = nd - 1; a >= 0; a--) {
+        double s = y[a];
+        for (int q = a + 1; q < nd; q++) s -= L[q][a] * grad[q];
+        grad[a] = s / L[a][a];
+    }
+    for (int a = nd; a < 3; a++) grad[a] = 0.0;
+}
+
 /* ---------------------------------------------------------------------------------------- */
 /* Reconstruction objects (src/Reconstruction, utility/Limiters.h)                           */
 /* ---------------------------------------------------------------------------------------- */
@@ -540,6 +579,7 @@ struct tso_ctx {
     std::vector<double> C[3], H[3];
     std::vector<double> vol;
     std::vector<double> U, R, SR, U0, Ust;
+    std::vector<double> G; /* velocity gradients, 9 per cell (ghosts included): [u,v,w][x,y,z] */
     std::vector<Rec> rec;
     int rl[3], rn[3]; /* recon layers and extents */
     double time = 0;
@@ -704,8 +744,152 @@ struct tso_ctx {
         }
     }
 
-    /* ComputeResidual (ERK:1150-1424), inviscid */
-    void residual(std::vector<double>& Rv, std::vector<double>& SRv) {
+    double vel(const std::vector<double>& V, size_t c, int d) const { return V[c * NV + 1 + d] / V[c * NV]; }
+
+    /* ComputeVelocityGradients (ERK:79-132): central differences in the interior
+       (ComputeGradientNonUniformStructured, kernel.h:918-938), the periodic exchange
+       (ExchangeGradients, ERK:135-300) and the dummy-cell gradients of the walls
+       (ComputeDummyCellGradients, ERK:303-665). V has its ghost layers filled. */
+    void velocity_gradients(const std::vector<double>& V) {
+        G.assign(vol.size() * 9, 0.0);
+        for (int k = gl[2]; k < gl[2] + n[2]; k++)
+            for (int j = gl[1]; j < gl[1] + n[1]; j++)
+                for (int i = gl[0]; i < gl[0] + n[0]; i++) {
+                    int I[3] = {i, j, k};
+                    size_t c = idx(i, j, k);
+                    for (int a = 0; a < nd; a++) {
+                        int P[3] = {i, j, k}, M[3] = {i, j, k};
+                        P[a]++;
+                        M[a]--;
+                        size_t cp = idx(P[0], P[1], P[2]), cm = idx(M[0], M[1], M[2]);
+                        double dX = C[a][I[a] + 1] - C[a][I[a] - 1];
+                        for (int d = 0; d < 3; d++) G[c * 9 + d * 3 + a] = (vel(V, cp, d) - vel(V, cm, d)) / dX;
+                    }
+                }
+        for (int a = 0; a < nd; a++) {
+            int ja = (a == 0) ? 1 : 0, ka = (a == 2) ? 1 : 2;
+            int nj = ja < nd ? n[ja] : 1, nk = ka < nd ? n[ka] : 1;
+            int lo = gl[a], hi = gl[a] + n[a] - 1;
+            for (int kk = 0; kk < nk; kk++)
+                for (int jj = 0; jj < nj; jj++)
+                    for (int side = 0; side < 2; side++) {
+                        int I[3], Gc[3];
+                        I[ja] = Gc[ja] = jj + gl[ja];
+                        I[ka] = Gc[ka] = kk + gl[ka];
+                        I[a] = side ? hi : lo;
+                        Gc[a] = side ? hi + 1 : lo - 1;
+                        size_t c = idx(I[0], I[1], I[2]), gc = idx(Gc[0], Gc[1], Gc[2]);
+                        if (cfg.periodic[a]) {
+                            /* single-rank exchange: the ghost takes the gradient of the wrapped cell */
+                            int S[3] = {I[0], I[1], I[2]};
+                            S[a] = side ? lo : hi;
+                            size_t sc = idx(S[0], S[1], S[2]);
+                            for (int q = 0; q < 9; q++) G[gc * 9 + q] = G[sc * 9 + q];
+                            continue;
+                        }
+                        /* face centre, boundary value (BCGeneral.h:90-94) and the T-stencil of
+                           Stencil::InternalBorderStencil (Stencil.h:28-61) plus the dummy cell */
+                        double fc[3] = {C[0][I[0]], C[1][I[1]], C[2][I[2]]};
+                        fc[a] = side ? C[a][I[a]] + 0.5 * H[a][I[a]] : C[a][I[a]] - 0.5 * H[a][I[a]];
+                        double nrm[3] = {0, 0, 0};
+                        nrm[a] = side ? 1.0 : -1.0;
+                        int m = marker(2 * a + side, jj, kk);
+                        double dummy[NV], Vf[NV];
+                        bc_image(cfg.bc[m], cfg.gamma, &V[c * NV], nrm, dummy);
+                        for (int v = 0; v < NV; v++) Vf[v] = 0.5 * (V[c * NV + v] + dummy[v]);
+                        int cells[6][3];
+                        int ns = 0;
+                        auto push = [&](int di, int dj, int dk) {
+                            cells[ns][0] = I[0] + di;
+                            cells[ns][1] = I[1] + dj;
+                            cells[ns][2] = I[2] + dk;
+                            ns++;
+                        };
+                        push(0, 0, 0);
+                        if (a == 0) {
+                            if (nd > 1) { push(0, 1, 0); push(0, -1, 0); }
+                            if (nd > 2) { push(0, 0, 1); push(0, 0, -1); }
+                        } else if (a == 1) {
+                            push(1, 0, 0); push(-1, 0, 0);
+                            if (nd > 2) { push(0, 0, 1); push(0, 0, -1); }
+                        } else {
+                            push(1, 0, 0); push(-1, 0, 0); push(0, 1, 0); push(0, -1, 0);
+                        }
+                        cells[ns][0] = Gc[0];
+                        cells[ns][1] = Gc[1];
+                        cells[ns][2] = Gc[2];
+                        ns++;
+                        double pts[6][3], vals[6];
+                        for (int q = 0; q < ns; q++)
+                            for (int e = 0; e < 3; e++) pts[q][e] = C[e][cells[q][e]];
+                        for (int d = 0; d < 3; d++) {
+                            for (int q = 0; q < ns; q++) vals[q] = vel(V, idx(cells[q][0], cells[q][1], cells[q][2]), d);
+                            double gf[3];
+                            lsq_gradient(nd, fc, Vf[1 + d] / Vf[0], pts, vals, ns, gf);
+                            for (int e = 0; e < 3; e++) G[gc * 9 + d * 3 + e] = 2.0 * gf[e] - G[c * 9 + d * 3 + e];
+                        }
+                    }
+        }
+    }
+
+    /* viscous flux through the face between cells cL and cR = cL + e_a, the right cell at index
+       i along a (ComputeViscousFluxes, ERK:668-874): face-normal derivatives from the two cell
+       values, tangential ones from the distance-weighted cell gradients */
+    void viscous_flux(const std::vector<double>& V, int a, size_t cL, size_t cR, int i, double* fv) const {
+        const double mu = cfg.viscosity;
+        const double s_visc = (-2.0 / 3.0) * mu;
+        double dx = C[a][i] - C[a][i - 1];
+        double l = H[a][i - 1] / dx;
+        double r = H[a][i] / dx;
+        double D[3][3]; /* D[d][e] = d(vel_d)/dx_e at the face */
+        double uf[3];
+        for (int d = 0; d < 3; d++) {
+            for (int e = 0; e < 3; e++) D[d][e] = 0.5 * (r * G[cL * 9 + d * 3 + e] + l * G[cR * 9 + d * 3 + e]);
+            double vl = vel(V, cL, d), vr = vel(V, cR, d);
+            D[d][a] = (vr - vl) / dx;
+            uf[d] = 0.5 * (l * vr + r * vl);
+        }
+        double tau_d = s_visc * (D[0][0] + D[1][1] + D[2][2]);
+        double t[3]; /* row a of the stress tensor */
+        for (int d = 0; d < 3; d++)
+            t[d] = (d == a) ? tau_d + 2 * mu * D[a][a] : mu * (D[std::min(a, d)][std::max(a, d)] + D[std::max(a, d)][std::min(a, d)]);
+        fv[0] = 0.0;
+        fv[1] = t[0];
+        fv[2] = t[1];
+        fv[3] = t[2];
+        fv[4] = uf[0] * t[0] + uf[1] * t[1] + uf[2] * t[2];
+    }
+
+    /* ProcessExternalForces / ProcessExternalAcceleration (kernel.h:941-989), after the sweeps */
+    void sources(const std::vector<double>& V, std::vector<double>& Rv) const {
+        if (!cfg.external_force && !cfg.external_accel) return;
+        for (int k = gl[2]; k < gl[2] + n[2]; k++)
+            for (int j = gl[1]; j < gl[1] + n[1]; j++)
+                for (int i = gl[0]; i < gl[0] + n[0]; i++) {
+                    size_t c = idx(i, j, k);
+                    double volume = H[0][i] * H[1][j] * H[2][k];
+                    double u = vel(V, c, 0), v = vel(V, c, 1), w = vel(V, c, 2);
+                    double* r = &Rv[c * NV];
+                    if (cfg.external_force) {
+                        const double* s = cfg.sigma;
+                        r[1] += volume * s[0];
+                        r[2] += volume * s[1];
+                        r[3] += volume * s[2];
+                        r[4] += (s[0] * u + s[1] * v + s[2] * w) * volume;
+                    }
+                    if (cfg.external_accel) {
+                        const double* g = cfg.accel;
+                        double ro = V[c * NV];
+                        r[1] += ro * volume * g[0];
+                        r[2] += ro * volume * g[1];
+                        r[3] += ro * volume * g[2];
+                        r[4] += (ro * g[0] * u + ro * g[1] * v + ro * g[2] * w) * volume;
+                    }
+                }
+    }
+
+    /* ComputeResidual (ERK:1150-1424) */
+    void residual(const std::vector<double>& V, std::vector<double>& Rv, std::vector<double>& SRv) {
         std::fill(Rv.begin(), Rv.end(), 0.0);
         std::fill(SRv.begin(), SRv.end(), 0.0);
         double g = cfg.gamma;
@@ -757,6 +941,15 @@ struct tso_ctx {
                     double lam;
                     flux(cfg.riemann, g, cfg.eps, UL, UR, fn, fr, &lam);
                     int I[3] = {T[0], T[1], T[2]};
+                    if (cfg.viscous) {
+                        /* fr -= fvisc (ERK:1221-1222) */
+                        I[a] = i - 1;
+                        size_t cL = idx(I[0], I[1], I[2]);
+                        I[a] = i;
+                        double fv[NV];
+                        viscous_flux(V, a, cL, idx(I[0], I[1], I[2]), i, fv);
+                        for (int v = 0; v < NV; v++) fr[v] -= fv[v];
+                    }
                     I[a] = i;
                     SRv[idx(I[0], I[1], I[2])] += fS * lam;
                     if (i > lo) {
@@ -764,10 +957,11 @@ struct tso_ctx {
                         size_t id = idx(I[0], I[1], I[2]);
                         for (int v = 0; v < NV; v++) Rv[id * NV + v] -= (fr[v] - fl[v]) * fS;
                         SRv[id] += fS * lam;
-                        /* viscous spectral-radius term with zero viscosity (ERK:1239-1248) */
+                        /* viscous spectral-radius term (ERK:1239-1248): gas viscosity, Prandtl
+                           number 1, whether or not the viscous fluxes are on */
                         double roFace = std::sqrt(UL[0] * UR[0]);
                         double vsr = std::max(4.0 / (3.0 * roFace), g / roFace);
-                        vsr *= 0.0 / 1.0;
+                        vsr *= cfg.viscosity / 1.0;
                         vsr *= fS * fS;
                         vsr /= vol[id];
                         SRv[id] += vsr;
@@ -776,6 +970,7 @@ struct tso_ctx {
                 }
             });
         }
+        sources(V, Rv);
     }
 
     double compute_dt(const std::vector<double>& SRv, double next_snap) {
@@ -796,7 +991,8 @@ struct tso_ctx {
         std::vector<double>& Vm = const_cast<std::vector<double>&>(Vin);
         fill_ghosts(Vm);
         reconstruct_all(Vin);
-        residual(Rv, SRv);
+        if (cfg.viscous) velocity_gradients(Vin);
+        residual(Vin, Rv, SRv);
     }
 
     int step(double next_snap, double* dt_out, double* norms) {
@@ -985,6 +1181,20 @@ void tso_reconstruct(int recon, int limiter, int ndims, const double* h, const d
 void tso_bc_image(const ts_bc_config* bc, double gamma, const double* U, const double* n,
                   double* out) {
     bc_image(*bc, gamma, U, n, out);
+}
+
+int tso_velocity_gradients(tso_ctx* o, double* grads, size_t n) {
+    if (n != o->vol.size() * 9) return TS_ERR_INVALID;
+    std::vector<double> V = o->U;
+    o->fill_ghosts(V);
+    o->velocity_gradients(V);
+    std::memcpy(grads, o->G.data(), n * sizeof(double));
+    return TS_OK;
+}
+
+void tso_lsq_gradient(int ndims, const double* point, double value, const double* points,
+                      const double* values, int m, double* grad) {
+    lsq_gradient(ndims, point, value, reinterpret_cast<const double (*)[3]>(points), values, m, grad);
 }
 
 }  // extern "C"

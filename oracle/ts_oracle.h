@@ -48,6 +48,12 @@ void tso_reconstruct(int recon, int limiter, int ndims, const double* h, const d
 /* boundary image of a state (getDummyValues) */
 void tso_bc_image(const ts_bc_config* bc, double gamma, const double* U, const double* n,
                   double* out);
+/* velocity gradients of the current state (ghost layers filled first), 9 per cell of the
+   reference layout: grads[cell*9 + d*3 + e] = d(vel_d)/dx_e, ghosts included */
+int tso_velocity_gradients(tso_ctx* o, double* grads, size_t n);
+/* least-squares gradient (GradientComputer::ExecuteByLeastSquares): points is m x 3 */
+void tso_lsq_gradient(int ndims, const double* point, double value, const double* points,
+                      const double* values, int m, double* grad);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,205 @@
+"""Pins of the oracle's viscous terms and body forces (CPU).
+
+The least-squares gradient against numpy's SVD least squares (the method class of the
+reference's Eigen JacobiSVD, GradientComputer.h:19-53); the dummy-cell gradients of a wall
+against the exact gradient of a field whose wall images are linear; the viscous face fluxes
+against closed forms of the discrete stress divergence and viscous heating (ERK:668-874,
+fluxes subtracted from the convective ones at ERK:1221-1222); the viscous spectral-radius term
+(ERK:1239-1248); the force and acceleration sources (kernel.h:941-989); and Poiseuille flow,
+whose centre velocity sigma H^2 / (8 mu) is the reference's own check (PoiseuilleChanel.h:44-47).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from turbostructred_amd import capi
+
+G = 1.4
+
+
+def centres(cfg):
+    o = po.Oracle(cfg)
+    shp = capi.state_shape(cfg)
+    X = o.grid(0, shp[2])[0]
+    Y = o.grid(1, shp[1])[0] if cfg.ndims > 1 else np.zeros(1)
+    Z = o.grid(2, shp[0])[0] if cfg.ndims > 2 else np.zeros(1)
+    return o, np.meshgrid(Z, Y, X, indexing="ij")
+
+
+def state(rho, u, v, w, p):
+    U = np.empty(rho.shape + (5,))
+    U[..., 0] = rho
+    U[..., 1] = rho * u
+    U[..., 2] = rho * v
+    U[..., 3] = rho * w
+    U[..., 4] = p / (G - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return U
+
+
+def residual_of(cfg, U):
+    o = po.Oracle(cfg)
+    o.set_state(U.ravel())
+    o.step(1)
+    r, sr = o.last_residual()
+    return r.reshape(U.shape), sr.reshape(U.shape[:-1])
+
+
+@pytest.mark.parametrize("nd,m", [(1, 2), (2, 4), (2, 5), (3, 6)])
+def test_lsq_gradient_matches_svd_least_squares(nd, m):
+    rng = np.random.default_rng(nd * 10 + m)
+    for _ in range(20):
+        pt = rng.normal(size=3)
+        pts = pt + rng.normal(scale=0.1, size=(m, 3))
+        vals = rng.normal(size=m)
+        val = rng.normal()
+        A = (pt - pts)[:, :nd]
+        ref = np.linalg.lstsq(A, val - vals, rcond=None)[0]
+        g = po.lsq_gradient(nd, pt, val, pts, vals)
+        np.testing.assert_allclose(g[:nd], ref, rtol=1e-11, atol=1e-12)
+        assert np.all(g[nd:] == 0)
+
+
+def test_lsq_gradient_exact_on_linear_data():
+    pt = np.array([0.3, -0.2, 0.1])
+    pts = pt + np.array([[-0.05, 0, 0], [-0.05, 0.1, 0], [-0.05, -0.1, 0], [0.05, 0, 0.02], [0, 0, -0.1]])
+    grad = np.array([1.5, -2.0, 0.25])
+    f = lambda q: 0.7 + q @ grad
+    g = po.lsq_gradient(3, pt, f(pt), pts, f(pts))
+    np.testing.assert_allclose(g, grad, rtol=1e-12)
+
+
+def test_wall_dummy_gradients_exact_for_mirror_linear_field():
+    """Symmetry walls at x = 0 and y = 0: u = b x and v = c y have linear mirror images, so the
+    least-squares face gradient and the dummy-cell extrapolation 2 g_face - g_in are exact."""
+    sym = dict(type=capi.BC_SYMMETRY)
+    cfg = capi.make_config(ndims=2, nx=12, ny=10, lx=1.2, ly=0.8, bcs=[sym], viscous=True, viscosity=0.1)
+    o, (Z, Y, X) = centres(cfg)
+    b, c = 0.7, -1.3
+    rho = 1.0 + 0.2 * X + 0.1 * Y
+    U = state(rho, b * X, c * Y, 0 * X, 1.0 + 0 * X)
+    o.set_state(U.ravel())
+    g = o.velocity_gradients().reshape(Z.shape + (3, 3))[0]
+    # bottom ghost row j = 0 (interior columns away from the x walls) and left ghost column i = 0
+    for gg in (g[0, 2:-3], g[2:-3, 0]):
+        np.testing.assert_allclose(gg[..., 0, 0], b, rtol=1e-12)
+        np.testing.assert_allclose(gg[..., 1, 1], c, rtol=1e-12)
+        np.testing.assert_allclose(gg[..., 0, 1], 0, atol=1e-12)
+        np.testing.assert_allclose(gg[..., 1, 0], 0, atol=1e-12)
+    # interior central differences are exact for linear fields on any grid (the cells next to
+    # the far walls see their non-linear mirror images)
+    np.testing.assert_allclose(g[1:-2, 1:-2, 0, 0], b, rtol=1e-12)
+    np.testing.assert_allclose(g[1:-2, 1:-2, 1, 1], c, rtol=1e-12)
+
+
+def test_natural_boundary_dummy_gradient_closed_form_1d():
+    """1-D, Natural boundary: the dummy cell copies the boundary cell, the face value equals the
+    cell value, so the face gradient is 0 and the dummy gradient is minus the inner one."""
+    cfg = capi.make_config(ndims=1, nx=16, viscous=True, viscosity=0.1)
+    o, (Z, Y, X) = centres(cfg)
+    U = state(1.0 + 0 * X, np.sin(3 * X), 0 * X, 0 * X, 1.0 + 0 * X)
+    o.set_state(U.ravel())
+    g = o.velocity_gradients().reshape(-1, 3, 3)
+    assert g[0, 0, 0] == -g[1, 0, 0] and g[1, 0, 0] != 0
+    assert g[-1, 0, 0] == -g[-2, 0, 0]
+
+
+def test_couette_flow_closed_form():
+    """Plane Couette flow u = y between a wall (y = 0) and a wall moving at u = Ly: the shear
+    stress is mu everywhere, so the momentum residual vanishes and the energy residual is the
+    viscous heating mu (du/dy)^2 V in every cell, the two wall rows included."""
+    mu, ly = 0.05, 0.5
+    bcs = [dict(type=capi.BC_WALL), dict(type=capi.BC_MOVING_WALL, velocity=(ly, 0, 0))]
+    cfg = capi.make_config(ndims=2, nx=6, ny=10, lx=0.3, ly=ly, periodic=(1, 0, 0), bcs=bcs,
+                           side_marker=(0, 0, 0, 1, 0, 0), viscous=True, viscosity=mu)
+    o, (Z, Y, X) = centres(cfg)
+    U = state(1.0 + 0 * X, Y.copy(), 0 * X, 0 * X, 2.0 + 0 * X)
+    r, _ = residual_of(cfg, U)
+    V = 0.05 * 0.05
+    ri = r[0, 1:-1, 1:-1]
+    np.testing.assert_allclose(ri[..., 1:4], 0, atol=1e-15)
+    np.testing.assert_allclose(ri[..., 4], mu * V, rtol=1e-11)
+
+
+def test_stress_divergence_and_heating_closed_form():
+    """u = 0, v = x y, w = x on a uniform grid: the discrete viscous residual of an interior cell
+    is (0, mu V / 3, 0, 0, V (mu y^2 + 4/3 mu x^2 + mu)) -- normal, tangential and bulk terms."""
+    mu = 0.1
+    kw = dict(ndims=2, nx=10, ny=8, lx=1.0, ly=0.8)
+    cfg_v = capi.make_config(viscous=True, viscosity=mu, **kw)
+    cfg_i = capi.make_config(**kw)
+    _, (Z, Y, X) = centres(cfg_v)
+    U = state(1.0 + 0 * X, 0 * X, X * Y, X.copy(), 3.0 + 0 * X)
+    rv, _ = residual_of(cfg_v, U)
+    ri, _ = residual_of(cfg_i, U)
+    d = (rv - ri)[0, 3:-3, 3:-3]
+    x, y = X[0, 3:-3, 3:-3], Y[0, 3:-3, 3:-3]
+    V = 0.1 * 0.1
+    np.testing.assert_allclose(d[..., 0], 0, atol=1e-16)
+    np.testing.assert_allclose(d[..., 1], mu * V / 3, rtol=1e-9)
+    np.testing.assert_allclose(d[..., 2:4], 0, atol=1e-13)
+    np.testing.assert_allclose(d[..., 4], V * (mu * y * y + 4 / 3 * mu * x * x + mu), rtol=1e-9)
+
+
+def test_viscous_spectral_radius_term():
+    """sum over a cell's plus faces of max(4/(3 rho), gamma/rho) mu fS^2 / V, added whether or
+    not the viscous fluxes are on (the gas viscosity is used, ERK:1239-1248)."""
+    mu, rho = 0.2, 1.3
+    kw = dict(ndims=2, nx=6, ny=5, lx=0.6, ly=1.0, periodic=(1, 1, 0))
+    _, (Z, Y, X) = centres(capi.make_config(**kw))
+    U = state(rho + 0 * X, 0.3 + 0 * X, -0.1 + 0 * X, 0 * X, 1.0 + 0 * X)
+    _, s0 = residual_of(capi.make_config(**kw), U)
+    _, s1 = residual_of(capi.make_config(viscosity=mu, **kw), U)
+    hx, hy = 0.1, 0.2
+    want = G / rho * mu * (hy * hy + hx * hx) / (hx * hy)
+    np.testing.assert_allclose((s1 - s0)[0, 1:-1, 1:-1], want, rtol=1e-12)
+
+
+def test_force_and_acceleration_sources_closed_form():
+    sig, acc = (2.0, -0.5, 0.25), (0.0, -9.8, 1.0)
+    cfg = capi.make_config(ndims=2, nx=4, ny=6, lx=0.4, ly=0.3, periodic=(1, 1, 0), sigma=sig, accel=acc)
+    _, (Z, Y, X) = centres(cfg)
+    rho, vel = 1.7, np.array([0.3, -0.2, 0.1])
+    U = state(rho + 0 * X, vel[0] + 0 * X, vel[1] + 0 * X, vel[2] + 0 * X, 2.0 + 0 * X)
+    r, _ = residual_of(cfg, U)
+    V = 0.1 * 0.05
+    want = np.zeros(5)
+    want[1:4] = V * np.array(sig) + rho * V * np.array(acc)
+    want[4] = V * (np.dot(sig, vel) + rho * np.dot(acc, vel))
+    np.testing.assert_allclose(r[0, 1:-1, 1:-1], np.broadcast_to(want, (6, 4, 5)), rtol=1e-13, atol=1e-18)
+
+
+def test_viscous_periodic_conservation():
+    """Periodic box, no sources: viscous fluxes telescope, so mass, momentum and energy totals
+    stay constant to round-off."""
+    cfg = capi.make_config(ndims=2, nx=16, ny=12, lx=1.0, ly=0.75, periodic=(1, 1, 0),
+                           recon=capi.RECON_LINEAR_LIM, viscous=True, viscosity=0.02)
+    o, (Z, Y, X) = centres(cfg)
+    u = 0.3 * np.sin(2 * np.pi * Y / 0.75)
+    v = 0.2 * np.cos(2 * np.pi * X)
+    U = state(1.0 + 0.1 * np.sin(2 * np.pi * X), u, v, 0.05 * np.sin(2 * np.pi * (X + Y)), 1.0 + 0 * X)
+    o.set_state(U.ravel())
+    tot0 = U[0, 1:-1, 1:-1].sum(axis=(0, 1))
+    o.step(20)
+    U1 = o.get_state().reshape(U.shape)
+    tot1 = U1[0, 1:-1, 1:-1].sum(axis=(0, 1))
+    np.testing.assert_allclose(tot1, tot0, rtol=1e-13, atol=1e-13)
+
+
+def test_poiseuille_centre_velocity():
+    """Channel between two walls, periodic along x, driven by the force sigma: the steady
+    centre velocity is sigma H^2 / (8 mu) (PoiseuilleChanel.h:44-47)."""
+    sigma, mu, H = 2.0, 0.1, 0.5
+    bcs = [dict(type=capi.BC_WALL)]
+    cfg = capi.make_config(ndims=2, nx=4, ny=24, lx=0.1, ly=H, periodic=(1, 0, 0), bcs=bcs,
+                           cfl=0.45, recon=capi.RECON_LINEAR_LIM, viscous=True, viscosity=mu,
+                           sigma=(sigma, 0, 0))
+    o, (Z, Y, X) = centres(cfg)
+    U = state(1.0 + 0 * X, 0 * X, 0 * X, 0 * X, 100.0 + 0 * X)
+    o.set_state(U.ravel())
+    info = o.step(200000, t_max=2.5)
+    U1 = o.get_state().reshape(U.shape)[0, 1:-1, 2]
+    u = U1[:, 1] / U1[:, 0]
+    uc = sigma * H * H / (8 * mu)
+    yc = Y[0, 1:-1, 2]
+    np.testing.assert_allclose(u, 4 * uc * yc * (H - yc) / H**2, rtol=0.02, atol=0.01 * uc)
+    assert abs(u[11:13].mean() / uc - 1) < 0.01

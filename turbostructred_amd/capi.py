@@ -68,6 +68,12 @@ class Config(C.Structure):
         ("side_marker", C.c_int * 6),
         ("face_marker", C.POINTER(C.c_uint8) * 6),
         ("device", C.c_int),
+        ("viscous", C.c_int),
+        ("viscosity", C.c_double),
+        ("external_force", C.c_int),
+        ("sigma", C.c_double * 3),
+        ("external_accel", C.c_int),
+        ("accel", C.c_double * 3),
     ]
 
 
@@ -93,7 +99,8 @@ class BatchConfig(C.Structure):
 def make_config(ndims=1, nx=1, ny=1, nz=1, lx=1.0, ly=1.0, lz=1.0, periodic=(0, 0, 0),
                 gamma=1.4, cfl=0.4, eps=0.0, riemann=RIEMANN_ROE, recon=RECON_PWC,
                 limiter=LIMITER_BJ, rk_stages=1, precision=FP64, bcs=None, side_marker=None,
-                face_marker=None, device=0, coord=None, h=None):
+                face_marker=None, device=0, coord=None, h=None, viscous=False, viscosity=0.0,
+                sigma=None, accel=None):
     """Build a Config.  bcs: list of dicts {type, velocity, pstatic, density} (marker table);
     default is one Natural condition on every side.  face_marker: optional list of 6 uint8
     numpy arrays (or None) -- kept alive on the returned object."""
@@ -132,6 +139,12 @@ def make_config(ndims=1, nx=1, ny=1, nz=1, lx=1.0, ly=1.0, lz=1.0, periodic=(0, 
                     keep.append(arr)
                     getattr(c, name)[a] = arr.ctypes.data_as(C.POINTER(C.c_double))
     c.device = device
+    c.viscous, c.viscosity = int(viscous), float(viscosity)
+    c.external_force = int(sigma is not None)
+    c.external_accel = int(accel is not None)
+    for a in range(3):
+        c.sigma[a] = sigma[a] if sigma is not None else 0.0
+        c.accel[a] = accel[a] if accel is not None else 0.0
     c._keep = keep
     return c
 

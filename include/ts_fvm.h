@@ -106,9 +106,11 @@ typedef struct ts_config {
     int side_marker[6];
     const uint8_t* face_marker[6];
     int device;          /* HIP device ordinal */
-    /* viscous terms and body forces (KernelConfiguration.h:41-51, 86-88).  FP64 only. */
+    /* viscous terms and body forces (KernelConfiguration.h:41-51, 86-88).  FP64, 1-D and 2-D;
+       single solves only (a batch refuses them). */
     int viscous;         /* IsViscousFlow: viscous fluxes from cell velocity gradients (ERK:79-874) */
-    double viscosity;    /* Viscosity; also enters the viscous spectral-radius term (ERK:1239-1248) */
+    double viscosity;    /* Viscosity; enters the viscous spectral-radius term (ERK:1239-1248)
+                            whether or not `viscous` is set, as in the reference */
     int external_force;  /* IsExternalForceRequared: residual += V Sigma, V Sigma.u (kernel.h:941-965) */
     double sigma[3];     /* Sigma: potential force, e.g. a pressure gradient */
     int external_accel;  /* IsUnifromAccelerationRequared: residual += rho V g, rho V g.u (kernel.h:966-989) */
@@ -171,7 +173,7 @@ int ts_get_dt_log(ts_ctx* ctx, double* out, long long capacity, long long* n_out
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch.
    kind: 0 = dt pass, 1 = stage 1 / Euler stage, 2 = stage 2, 3 = stage 3, 4 = kinds 0-3,
-   5 = residual-norm pass. */
+   5 = residual-norm pass, 6 = viscous pre-pass (velocity gradients and viscous face fluxes). */
 int ts_enable_kernel_timing(ts_ctx* ctx, int enable);
 int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* launches);
 

@@ -1,0 +1,161 @@
+"""GPU parity of the viscous path and the body forces (ERK:79-874 gradients and viscous fluxes,
+ERK:1239-1248 viscous spectral radius, kernel.h:941-989 sources) against the oracle, and the
+Poiseuille centre velocity sigma H^2 / (8 mu) of PoiseuilleChanel.h:44-47 on the device."""
+import numpy as np
+import pytest
+
+from turbostructred_amd import capi
+from oracle import pyoracle
+import util
+from test_gpu_parity import run_pair, check, TOL
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = [capi.RIEMANN_ROE, capi.RIEMANN_HLLC, capi.RIEMANN_GODUNOV]
+RECONS = [(capi.RECON_PWC, 0), (capi.RECON_ENO2, 0), (capi.RECON_LINEAR_LIM, capi.LIMITER_BJ),
+          (capi.RECON_LINEAR_LIM, capi.LIMITER_VENKAT)]
+
+
+def shear_1d(g):
+    def f(x, y, z):
+        rho = 1 + 0.3 * np.sin(2 * np.pi * x)
+        return util.prim2cons(rho, 0.4 * np.cos(2 * np.pi * x), 0.3 * np.sin(4 * np.pi * x),
+                              -0.2 * np.cos(6 * np.pi * x), 1 + 0.1 * np.sin(2 * np.pi * x), g)
+    return f
+
+
+def shear_2d(g):
+    def f(x, y, z):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+        u = 0.3 * np.sin(np.pi * y) + 0.1 * np.cos(2 * np.pi * x)
+        v = -0.2 * np.sin(2 * np.pi * x) * np.sin(np.pi * y)
+        w = 0.1 * np.cos(2 * np.pi * (x + y))
+        p = 1 + 0.1 * np.cos(2 * np.pi * (x - y))
+        return util.prim2cons(rho, u, v, w, p, g)
+    return f
+
+
+MIXED_BCS = [dict(type=capi.BC_WALL), dict(type=capi.BC_MOVING_WALL, velocity=(0.3, -0.1, 0.2)),
+             dict(type=capi.BC_SYMMETRY), dict(type=capi.BC_NATURAL),
+             dict(type=capi.BC_SUBSONIC_INLET, velocity=(0.2, 0.1, 0.0), pstatic=1.0, density=1.1)]
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_1d_viscous_matrix(recon, lim, flux, stages):
+    cfg = capi.make_config(ndims=1, nx=157, cfl=0.4, riemann=flux, recon=recon, limiter=lim, rk_stages=stages,
+                           eps=0.1 if flux == capi.RIEMANN_ROE else 0.0, bcs=MIXED_BCS,
+                           side_marker=(0, 1, 0, 0, 0, 0), viscous=True, viscosity=0.01,
+                           sigma=(0.5, -0.2, 0.1), accel=(0.3, 0.0, -0.4))
+    U = util.fill_state(cfg, shear_1d(cfg.gamma))
+    check(*run_pair(cfg, U, 60))
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_2d_viscous_matrix(recon, lim, flux, stages):
+    """every side a different condition (wall, moving wall, symmetry, subsonic inlet)"""
+    cfg = capi.make_config(ndims=2, nx=131, ny=70, ly=0.8, cfl=0.45, riemann=flux, recon=recon, limiter=lim,
+                           rk_stages=stages, bcs=MIXED_BCS, side_marker=(2, 4, 0, 1, 0, 0),
+                           viscous=True, viscosity=0.005, sigma=(1.0, 0.0, 0.2), accel=(0.0, -1.0, 0.0))
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    check(*run_pair(cfg, U, 25))
+
+
+@pytest.mark.parametrize("periodic", [(1, 0, 0), (0, 1, 0), (1, 1, 0)])
+@pytest.mark.parametrize("stages", [1, 3])
+def test_2d_viscous_periodic(periodic, stages):
+    cfg = capi.make_config(ndims=2, nx=64, ny=50, cfl=0.45, periodic=periodic, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=stages, bcs=MIXED_BCS,
+                           side_marker=(0, 1, 1, 0, 0, 0), viscous=True, viscosity=0.02)
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    check(*run_pair(cfg, U, 30))
+
+
+def stretched(n, L, q):
+    """two geometric blocks meeting at L/2 (ratio q, then 1/q), ghost sizes continuing the
+    progression: cell centres and sizes, ghost layer included"""
+    half = n // 2
+    hb = q ** np.arange(half)
+    hb *= 0.5 * L / hb.sum()
+    h = np.concatenate([[hb[0] / q], hb, hb[::-1], [hb[0] / q]])
+    faces = np.concatenate([[0.0], np.cumsum(h[1:-1])])
+    x = np.empty(n + 2)
+    x[1:-1] = 0.5 * (faces[:-1] + faces[1:])
+    x[0] = x[1] - 0.5 * (h[0] + h[1])
+    x[-1] = x[-2] + 0.5 * (h[-2] + h[-1])
+    return x, h
+
+
+@pytest.mark.parametrize("stages", [1, 3])
+def test_2d_viscous_stretched_channel(stages):
+    """Poiseuille's grid: walls at y = 0 and y = Ly on a y-axis compressed towards both walls"""
+    yc, hy = stretched(40, 0.5, 1.05)
+    cfg = capi.make_config(ndims=2, nx=61, ny=40, lx=1.0, ly=0.5, periodic=(1, 0, 0), cfl=0.45,
+                           riemann=capi.RIEMANN_ROE, eps=0.05, recon=capi.RECON_LINEAR_LIM, rk_stages=stages,
+                           bcs=[dict(type=capi.BC_WALL)], viscous=True, viscosity=0.01, sigma=(2.0, 0, 0),
+                           coord=[None, yc, None], h=[None, hy, None])
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    check(*run_pair(cfg, U, 40))
+
+
+@pytest.mark.parametrize("viscous,sigma,accel", [(False, None, None), (False, (1.0, 0.5, 0.0), None),
+                                                 (False, None, (0.0, -2.0, 0.5))])
+def test_spectral_radius_and_sources_without_viscous_fluxes(viscous, sigma, accel):
+    """a viscosity alone changes dt through the spectral radius; forces alone need no pre-pass"""
+    cfg = capi.make_config(ndims=2, nx=70, ny=45, cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3, bcs=MIXED_BCS, side_marker=(0, 1, 2, 3, 0, 0),
+                           viscous=viscous, viscosity=0.05, sigma=sigma, accel=accel)
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    check(*run_pair(cfg, U, 30))
+
+
+@pytest.mark.parametrize("nd,stages", [(1, 1), (2, 1), (2, 3)])
+def test_viscous_residual_norms(nd, stages):
+    kw = dict(ndims=nd, nx=90, ny=40 if nd > 1 else 1, cfl=0.4, riemann=capi.RIEMANN_ROE,
+              recon=capi.RECON_LINEAR_LIM, rk_stages=stages, bcs=MIXED_BCS, side_marker=(0, 1, 2, 0, 0, 0),
+              viscous=True, viscosity=0.01, sigma=(0.3, 0.1, 0.0))
+    cfg = capi.make_config(**kw)
+    U = util.fill_state(cfg, (shear_1d if nd == 1 else shear_2d)(cfg.gamma))
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    gpu.enable_residual_norms(True)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U)
+    for k in (1, 7):
+        gi = gpu.step(k)
+        ri = ref.step(k)
+        np.testing.assert_allclose(np.array(gi.residual_norm[:5]), np.array(ri.residual_norm[:5]), rtol=1e-12)
+
+
+def test_poiseuille_centre_velocity_on_gpu():
+    """channel flow driven by sigma between two walls reaches u_c = sigma H^2 / (8 mu) and the
+    parabolic profile; the device run stays with the oracle over the whole run"""
+    sigma, mu, H = 2.0, 0.1, 0.5
+    cfg = capi.make_config(ndims=2, nx=4, ny=24, lx=0.1, ly=H, periodic=(1, 0, 0), bcs=[dict(type=capi.BC_WALL)],
+                           cfl=0.45, recon=capi.RECON_LINEAR_LIM, viscous=True, viscosity=mu,
+                           sigma=(sigma, 0, 0))
+    U = util.fill_state(cfg, lambda x, y, z: util.prim2cons(1.0 + 0 * x, 0 * x, 0 * x, 0 * x, 100.0 + 0 * x,
+                                                            cfg.gamma))
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 200000, t_max=2.5)
+    check(a, b, dg, dr, gi, ri, tol=1e-10)
+    col = a.reshape(24, 4, 5)[:, 2]
+    u = col[:, 1] / col[:, 0]
+    uc = sigma * H * H / (8 * mu)
+    y = util.centres(cfg, 1)[1:-1]
+    np.testing.assert_allclose(u, 4 * uc * y * (H - y) / H**2, rtol=0.02, atol=0.01 * uc)
+    assert abs(u[11:13].mean() / uc - 1) < 0.01
+
+
+def test_viscous_kernel_timing_kind():
+    cfg = capi.make_config(ndims=2, nx=64, ny=64, periodic=(1, 1, 0), rk_stages=3, recon=capi.RECON_LINEAR_LIM,
+                           viscous=True, viscosity=0.01)
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    s = capi.Solver(cfg)
+    s.set_state(U)
+    s.enable_kernel_timing(True)
+    s.step(5)
+    ms, n = s.kernel_timing(capi.KIND_VISCOUS)
+    assert n == 15 and ms > 0

@@ -30,7 +30,7 @@ BC_WALL, BC_SYMMETRY, BC_MOVING_WALL, BC_SUBSONIC_INLET, BC_SUBSONIC_OUTLET, BC_
 FP64, FP32_RESIDUAL = 0, 1
 MAX_BC = 8
 
-KIND_DT, KIND_STAGE1, KIND_STAGE2, KIND_STAGE3, KIND_ALL, KIND_NORMS = 0, 1, 2, 3, 4, 5
+KIND_DT, KIND_STAGE1, KIND_STAGE2, KIND_STAGE3, KIND_ALL, KIND_NORMS, KIND_VISCOUS = 0, 1, 2, 3, 4, 5, 6
 
 
 class BCConfig(C.Structure):
@@ -303,9 +303,6 @@ class Solver:
         n = C.c_longlong()
         self._check(self.lib.ts_get_dt_log(self.h, _dp(out), cap, C.byref(n)))
         return out[: n.value]
-
-    def enable_residual_norms(self, on=True):
-        self._check(self.lib.ts_enable_residual_norms(self.h, int(on)))
 
     def enable_kernel_timing(self, on=True):
         self._check(self.lib.ts_enable_kernel_timing(self.h, int(on)))

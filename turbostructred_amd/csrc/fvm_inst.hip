@@ -2,6 +2,9 @@
 // per pair with -DTS_DIM=<1|2|3> -DTS_FLUX=<0 Godunov|1 Roe|3 HLLC> so that the kernel families
 // build in parallel.
 #include "fvm_dispatch.hpp"
+#if TS_DIM != 3
+#include "fvm_viscous.hpp"
+#endif
 #if TS_DIM == 3 && __has_include("fvm_kernels3d.hpp")
 #include "fvm_kernels3d.hpp"
 #define TS_HAVE_3D 1
@@ -72,6 +75,21 @@ static int batched_occupancy() {
 }
 #endif
 
+#if TS_DIM != 3
+// viscous pre-pass: gradients of the interior cells, then the faces of every axis
+static void launch_vis_prepass(const KArgs& a, dim3, dim3, hipStream_t s) {
+    const unsigned nb = unsigned((a.plane + 255) / 256);
+    hipLaunchKernelGGL(vis_gradients<TS_DIM>, dim3(nb), dim3(256), 0, s, a);
+    unsigned nf = 0;
+    for (int ax = 0; ax < TS_DIM; ax++) nf = a.v.nface[ax] > nf ? a.v.nface[ax] : nf;
+    hipLaunchKernelGGL(vis_face_flux<TS_DIM>, dim3((nf + 255) / 256, TS_DIM), dim3(256), 0, s, a);
+}
+template <int RC, int LM, int MODE>
+static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
+    hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
+}
+#endif
+
 template <int RC, int LM>
 static void fill(int precision, KernelSet* ks) {
     if (precision == 0) {
@@ -89,6 +107,12 @@ static void fill(int precision, KernelSet* ks) {
         ks->bf2 = launch_batched<RC, LM, MODE_F2, double, double, double>;
         ks->bs3 = launch_batched<RC, LM, MODE_S3, double, double, double>;
         ks->bocc = batched_occupancy<RC, LM, double, double>;
+        ks->vpre = launch_vis_prepass;
+        ks->vdt = launch_vis<RC, LM, MODE_DT>;
+        ks->vs1 = launch_vis<RC, LM, MODE_S1>;
+        ks->vs2 = launch_vis<RC, LM, MODE_S2>;
+        ks->vs3 = launch_vis<RC, LM, MODE_S3>;
+        ks->vrn = launch_vis<RC, LM, MODE_RN>;
 #endif
     } else {
         ks->dt = launch<RC, LM, MODE_DT, float, double, double>;

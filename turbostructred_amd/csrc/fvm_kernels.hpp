@@ -72,8 +72,24 @@ struct GridArgs {
     const uint8_t* face_marker[6];
 };
 
+// Viscous terms and body forces (ERK:79-874, kernel.h:941-989), read only by the VIS instances
+// of the stage kernels and by the viscous pre-pass.
+struct VisArgs {
+    const double* x[3];  // cell-centre coordinates, ghost layer included (n + 2 entries)
+    double* grad;        // velocity gradients of the interior cells: plane d*nd + e = d vel_d / d x_e
+    double* fv[3];       // viscous fluxes of the faces normal to axis a: 4 planes (momentum x, y, z,
+                         // energy), faces numbered like cells with n_a + 1 along a
+    unsigned nface[3];   // faces normal to axis a (the plane size of fv[a])
+    double mu;           // gas viscosity (viscous fluxes and the spectral-radius term)
+    double sigma[3];     // potential force (IsExternalForceRequared)
+    double accel[3];     // uniform acceleration (IsUnifromAccelerationRequared)
+    int viscous, force, accel_on;
+    int gate_go;         // pre-pass: runs when the step runs (1) or while steps remain (0)
+};
+
 struct KArgs {
     GridArgs g;
+    VisArgs v;
     const void* uin;   // stencil input planes (TIn)
     const void* lin;   // L(Un) planes (TOut type), MODE_F2
     const double* un;  // U^n planes (SSP stages 2, 3)
@@ -256,7 +272,44 @@ template <int MODE> struct ModeTraits {
 // ---- one tile of the 1-D / 2-D stage: a strip of 60 columns marching over ty rows ----------------
 // block_min receives the tile's min(V/SR) (sr modes).  `tile` < 0 marks a wavefront without work
 // that still has to take part in a block reduction.
-template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+// max(4/(3 rho_f), gamma/rho_f) mu fS^2 / V: the viscous part of a cell's spectral radius from
+// one face, rho_f = sqrt(rho_L rho_R) of the face states (ERK:1239-1248)
+__device__ inline double viscous_sr(double rho_l, double rho_r, double gamma, double mu, double fS, double vol) {
+#pragma clang fp contract(off)
+    const double ro = __builtin_sqrt(rho_l * rho_r);
+    const double a = 4.0 / (3.0 * ro), b = gamma / ro;
+    double v = (a < b) ? b : a;
+    v *= mu / 1.0;
+    v *= fS * fS;
+    v /= vol;
+    return v;
+}
+
+// residual += force and acceleration sources of a cell (kernel.h:941-989)
+template <typename R>
+__device__ inline void add_sources(const VisArgs& v, const R (&u)[NV], double vol, R (&res)[NV]) {
+    const double ro = double(u[0]);
+    const double vx = double(u[1]) / ro, vy = double(u[2]) / ro, vz = double(u[3]) / ro;
+    if (v.force) {
+        const double* s = v.sigma;
+        res[1] = R(double(res[1]) + vol * s[0]);
+        res[2] = R(double(res[2]) + vol * s[1]);
+        res[3] = R(double(res[3]) + vol * s[2]);
+        res[4] = R(double(res[4]) + (s[0] * vx + s[1] * vy + s[2] * vz) * vol);
+    }
+    if (v.accel_on) {
+        const double* g = v.accel;
+        res[1] = R(double(res[1]) + ro * vol * g[0]);
+        res[2] = R(double(res[2]) + ro * vol * g[1]);
+        res[3] = R(double(res[3]) + ro * vol * g[2]);
+        res[4] = R(double(res[4]) + (ro * g[0] * vx + ro * g[1] * vy + ro * g[2] * vz) * vol);
+    }
+}
+
+// VIS instances (FP64) add the viscous face fluxes of the pre-pass (subtracted from the convective
+// flux before it is differenced, ERK:1221-1222), the viscous spectral-radius term after each
+// plus face's eigenvalue, and the body-force sources after the flux sweeps.
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0>
 __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min,
                                            double* rn = nullptr) {
     using MT = ModeTraits<MODE>;
@@ -319,8 +372,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     };
 
     // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
-    auto finish = [&](unsigned off, const R (&u)[NV], const R (&res)[NV], const double (&un)[NV],
+    auto finish = [&](unsigned off, const R (&u)[NV], const R (&res0)[NV], const double (&un)[NV],
                       double ivol, double vol) {
+        R res[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) res[v] = res0[v];
+        if constexpr (VIS) add_sources(a.v, u, vol, res);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             if constexpr (MODE == MODE_RN) {
@@ -364,12 +421,23 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         R F[NV];
         if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lam);
         else lam = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+        if constexpr (VIS && MT::flux) {
+            if (a.v.viscous) {
+                const unsigned fi = unsigned(xg < 0 ? 0 : (xg > nx ? nx : xg));
+#pragma unroll
+                for (int v = 1; v < NV; v++) F[v] -= R(ld_plane(a.v.fv[0], a.v.nface[0], v - 1, fi));
+            }
+        }
         if constexpr (MT::sr) {
             R lam_r = dpp_from_right(lam);
             double sr;
             {
 #pragma clang fp contract(off)
                 sr = double(Sx * lam) + double(Sx * lam_r);
+            }
+            if constexpr (VIS) {
+                const double rl = dpp_from_right(double(UL[0])), rr = dpp_from_right(double(UR[0]));
+                sr = sr + viscous_sr(rl, rr, g.gamma, a.v.mu, double(Sx), hx);
             }
             if (out_lane) block_min = dt_candidate(hx, sr);
             if (ghost_x_lane) block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1], double(Sx * lam)));
@@ -445,6 +513,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 R G[NV], lam;
                 if constexpr (MT::flux) face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
                 else lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+                if constexpr (VIS && MT::flux) {
+                    if (a.v.viscous) {
+                        const unsigned fi = unsigned(rc) * unsigned(nx) + unsigned(xs);
+#pragma unroll
+                        for (int v = 1; v < NV; v++) G[v] -= R(ld_plane(a.v.fv[1], a.v.nface[1], v - 1, fi));
+                    }
+                }
                 if constexpr (MT::sr) {
                     if (t >= 2) {
                         double sr;
@@ -452,6 +527,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 #pragma clang fp contract(off)
                             sr = srp + double(Sy * lam);
                         }
+                        if constexpr (VIS)
+                            sr = sr + viscous_sr(double(GL[0]), double(GR[0]), g.gamma, a.v.mu, double(Sy), hx * g.h[1][rc]);
                         if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc], sr));
                     }
                     if (rc == ny && out_lane)  // top ghost row: only this face (ERK:1305)
@@ -487,11 +564,25 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                     R F[NV], lamx;
                     if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lamx);
                     else lamx = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+                    if constexpr (VIS && MT::flux) {
+                        if (a.v.viscous) {
+                            const unsigned fi = unsigned(rcs) * unsigned(nx + 1) + unsigned(xg < 0 ? 0 : (xg > nx ? nx : xg));
+#pragma unroll
+                            for (int v = 1; v < NV; v++) F[v] -= R(ld_plane(a.v.fv[0], a.v.nface[0], v - 1, fi));
+                        }
+                    }
                     if constexpr (MT::sr) {
                         R lam_r = dpp_from_right(lamx);
                         {
 #pragma clang fp contract(off)
                             srp = double(Sx * lamx) + double(Sx * lam_r);
+                        }
+                        if constexpr (VIS) {
+                            const double rl = dpp_from_right(double(UL[0])), rr = dpp_from_right(double(UR[0]));
+                            srp = srp + viscous_sr(rl, rr, g.gamma, a.v.mu, double(Sx), hx * hy);
+                        }
+                        {
+#pragma clang fp contract(off)
                             srp = srp + double(Sy * lam_gb);
                         }
                         if (ghost_x_lane)  // right ghost column (ERK:1225)
@@ -518,7 +609,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 }
 
 // ---- single-solve kernel: one wavefront per tile, block-level dt reduction ----------------------
-template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0>
 __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
     using MT = ModeTraits<MODE>;
@@ -544,7 +635,7 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     }
     double block_min = __builtin_inf();
     double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
+    stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, VIS>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
     if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
     if constexpr (MODE == MODE_RN) {
         __shared__ double rred[4 * NV];

@@ -551,6 +551,8 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
     // the same validation as ts_create, then the batch's own limits
     if (cfg->ndims < 1 || cfg->ndims > 2) return fail(b, TS_ERR_UNSUPPORTED, "batched solves are 1-D or 2-D");
     if (cfg->rk_stages != 3) return fail(b, TS_ERR_UNSUPPORTED, "batched solves use SSP-RK3");
+    if (cfg->viscous || cfg->external_force || cfg->external_accel || cfg->viscosity != 0.0)
+        return fail(b, TS_ERR_UNSUPPORTED, "viscous terms and body forces are not batched");
     if (cfg->ghost != 1 || cfg->nx < 2 || (cfg->ndims > 1 && cfg->ny < 2) || cfg->nz != 1 ||
         (cfg->ndims == 1 && cfg->ny != 1) || !(cfg->gamma > 1.0) || !(cfg->cfl > 0.0) || cfg->n_bc < 1 ||
         cfg->n_bc > TS_MAX_BC)

@@ -50,8 +50,12 @@ struct ts_ctx {
 
     // kernel timing
     bool timing = false;
-    double t_ms[6] = {0, 0, 0, 0, 0, 0};  // per kind: dt, stages 1-3, (unused), residual norms
-    long long t_n[6] = {0, 0, 0, 0, 0, 0};
+    // per kind: dt, stages 1-3, (unused), residual norms, viscous pre-pass
+    double t_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long t_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    // viscous terms / body forces / viscous spectral radius: the VIS kernel instances run
+    bool vis = false;
+    tsd::VisArgs visargs{};
     unsigned* d_cells = nullptr;  // ts_get_cells: cell indices and gathered values
     double* d_cell_vals = nullptr;
     size_t cells_cap = 0;

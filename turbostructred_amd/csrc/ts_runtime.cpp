@@ -83,6 +83,16 @@ int validate(const ts_config* c, std::string& err) {
     if (c->n_bc < 0 || c->n_bc > TS_MAX_BC) { err = "n_bc out of range"; return TS_ERR_INVALID; }
     for (int m = 0; m < c->n_bc; m++)
         if (c->bc[m].type < TS_BC_WALL || c->bc[m].type > TS_BC_NATURAL) { err = "unknown BC type"; return TS_ERR_INVALID; }
+    if (c->viscous || c->external_force || c->external_accel || c->viscosity != 0.0) {
+        if (!(c->viscosity >= 0.0) || !(c->viscosity < 1e300)) { err = "viscosity must be finite and >= 0"; return TS_ERR_INVALID; }
+        for (int a = 0; a < 3; a++)
+            if (!(c->sigma[a] - c->sigma[a] == 0.0) || !(c->accel[a] - c->accel[a] == 0.0)) {
+                err = "Sigma and UniformAcceleration must be finite";
+                return TS_ERR_INVALID;
+            }
+        if (c->precision != TS_FP64) { err = "viscous terms and body forces run in FP64 only"; return TS_ERR_UNSUPPORTED; }
+        if (c->ndims > 2) { err = "viscous terms and body forces are built for 1-D and 2-D grids"; return TS_ERR_UNSUPPORTED; }
+    }
     for (int a = 0; a < c->ndims; a++) {
         if (c->periodic[a]) continue;
         for (int s = 0; s < 2; s++) {
@@ -294,12 +304,12 @@ int ts_get_dt_log(ts_ctx* c, double* out, long long capacity, long long* n_out) 
 int ts_enable_kernel_timing(ts_ctx* c, int enable) {
     if (!c) return TS_ERR_INVALID;
     c->timing = enable != 0;
-    for (int k = 0; k < 6; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    for (int k = 0; k < 7; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
     return TS_OK;
 }
 
 int ts_get_kernel_timing(ts_ctx* c, int kind, double* total_ms, long long* launches) {
-    if (!c || kind < 0 || kind > 5 || !total_ms || !launches) return TS_ERR_INVALID;
+    if (!c || kind < 0 || kind > 6 || !total_ms || !launches) return TS_ERR_INVALID;
     if (kind == 4) {
         double s = 0;
         long long n = 0;

@@ -145,6 +145,31 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->err = "no kernel instance for this (dimension, flux, reconstruction) combination";
         return TS_ERR_UNSUPPORTED;
     }
+    c->vis = k.viscous || k.external_force || k.external_accel || k.viscosity != 0.0;
+    if (c->vis) {
+        if (!c->ks.vdt || !c->ks.vpre) {
+            c->err = "no viscous kernel instance for this combination";
+            return TS_ERR_UNSUPPORTED;
+        }
+        VisArgs& v = c->visargs;
+        v.mu = k.viscosity;
+        v.viscous = k.viscous;
+        v.force = k.external_force;
+        v.accel_on = k.external_accel;
+        for (int a = 0; a < 3; a++) {
+            v.sigma[a] = k.sigma[a];
+            v.accel[a] = k.accel[a];
+        }
+        if (k.viscous) {
+            int r;
+            if ((r = dalloc(c, &v.grad, c->ncell * 3 * c->nd))) return r;
+            for (int a = 0; a < c->nd; a++) {
+                if ((r = upload(c, c->X[a].data(), c->X[a].size(), &v.x[a]))) return r;
+                v.nface[a] = (unsigned)(c->ncell / c->n[a] * (c->n[a] + 1));
+                if ((r = dalloc(c, &v.fv[a], (size_t)v.nface[a] * 4))) return r;
+            }
+        }
+    }
     // state buffers
     int r;
     double* u0 = nullptr;
@@ -187,7 +212,7 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     }
     c->host_stage.resize(c->ncell * 5);
     const char* sep = getenv("TS_RK3_SEPARATE_DT");
-    c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2;
+    c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
     return TS_OK;
 }
 
@@ -266,10 +291,13 @@ struct Launch {
     int kind;
 };
 
+constexpr int MAX_LAUNCHES = 8;  // per step: 3 viscous pre-passes, dt, norms, 3 stages
+
 // the launches of step s of a chunk, given the buffer holding U^n at the chunk start
-void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[5], int& n) {
+void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[MAX_LAUNCHES], int& n) {
     KArgs a{};
     a.g = c->grid;
+    a.v = c->visargs;
     a.plane = c->ncell;
     a.ctl = c->dctl;
     a.ty = c->ty;
@@ -285,8 +313,44 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[5], int& n
         r.uin = un_buf;
         r.lin = nullptr;
         r.uout = nullptr;
-        L[n++] = {c->ks.rn, r, 5};
+        L[n++] = {c->vis ? c->ks.vrn : c->ks.rn, r, 5};
     };
+    // viscous pre-pass over the stage input: while steps remain (before the dt pass), or when the
+    // step runs (later stages)
+    auto prepass = [&](const void* in, int gate_go) {
+        if (!c->cfg.viscous) return;
+        KArgs p = a;
+        p.uin = in;
+        p.v.gate_go = gate_go;
+        L[n++] = {c->ks.vpre, p, 6};
+    };
+    if (c->vis) {
+        if (c->cfg.rk_stages == 1) {
+            int in = (start_buf + s) & 1, out = in ^ 1;
+            a.uin = c->dU[in];
+            prepass(a.uin, 0);
+            L[n++] = {c->ks.vdt, a, 0};
+            norms(c->dU[in]);
+            a.uout = c->dU[out];
+            L[n++] = {c->ks.vs1, a, 1};
+        } else {
+            a.uin = c->dU[0];
+            prepass(a.uin, 0);
+            L[n++] = {c->ks.vdt, a, 0};
+            norms(c->dU[0]);
+            a.uout = c->dU[1];
+            L[n++] = {c->ks.vs1, a, 1};
+            a.uin = c->dU[1];
+            prepass(a.uin, 1);
+            a.uout = c->dU[2];
+            L[n++] = {c->ks.vs2, a, 2};
+            a.uin = c->dU[2];
+            prepass(a.uin, 1);
+            a.uout = c->dU[0];
+            L[n++] = {c->ks.vs3, a, 3};
+        }
+        return;
+    }
     if (c->cfg.rk_stages == 1) {
         int in = (start_buf + s) & 1, out = in ^ 1;
         a.uin = c->dU[in];
@@ -324,7 +388,7 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[5], int& n
 
 int issue_direct(ts_ctx* c, int start_buf, int first, int count) {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> rec;
-    size_t need = c->timing ? (size_t)count * 8 : 0;
+    size_t need = c->timing ? (size_t)count * 2 * MAX_LAUNCHES : 0;
     while (c->events.size() < need) {
         hipEvent_t e;
         CHK(hipEventCreate(&e));
@@ -332,7 +396,7 @@ int issue_direct(ts_ctx* c, int start_buf, int first, int count) {
     }
     size_t ev = 0;
     for (int s = first; s < first + count; s++) {
-        Launch L[5];
+        Launch L[MAX_LAUNCHES];
         int n;
         step_launches(c, start_buf, s, L, n);
         for (int i = 0; i < n; i++) {
@@ -370,7 +434,7 @@ int issue_graph(ts_ctx* c, int start_buf, int count) {
         hipGraph_t graph;
         CHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < count; s++) {
-            Launch L[5];
+            Launch L[MAX_LAUNCHES];
             int n;
             step_launches(c, start_buf, s, L, n);
             for (int i = 0; i < n; i++) L[i].fn(L[i].args, c->grid_dim, c->block_dim, c->stream);

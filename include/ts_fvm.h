@@ -106,8 +106,8 @@ typedef struct ts_config {
     int side_marker[6];
     const uint8_t* face_marker[6];
     int device;          /* HIP device ordinal */
-    /* viscous terms and body forces (KernelConfiguration.h:41-51, 86-88).  FP64, 1-D and 2-D;
-       single solves only (a batch refuses them). */
+    /* viscous terms and body forces (KernelConfiguration.h:41-51, 86-88).  FP64 only; single
+       solves only (a batch refuses them). */
     int viscous;         /* IsViscousFlow: viscous fluxes from cell velocity gradients (ERK:79-874) */
     double viscosity;    /* Viscosity; enters the viscous spectral-radius term (ERK:1239-1248)
                             whether or not `viscous` is set, as in the reference */

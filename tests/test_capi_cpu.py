@@ -68,14 +68,15 @@ def test_invalid_configs_rejected_before_device_use(bad):
 
 
 @pytest.mark.parametrize("kw,code", [
-    (dict(ndims=3, nx=4, ny=4, nz=4, viscous=True, viscosity=0.1), capi.TS_ERR_UNSUPPORTED),
+    (dict(ndims=3, nx=4, ny=4, nz=4, precision=capi.FP32_RESIDUAL, viscous=True, viscosity=0.1),
+     capi.TS_ERR_UNSUPPORTED),
     (dict(ndims=2, nx=8, ny=8, precision=capi.FP32_RESIDUAL, sigma=(1.0, 0, 0)), capi.TS_ERR_UNSUPPORTED),
     (dict(ndims=1, nx=8, viscosity=-1.0), capi.TS_ERR_INVALID),
     (dict(ndims=1, nx=8, viscous=True, viscosity=float("nan")), capi.TS_ERR_INVALID),
     (dict(ndims=1, nx=8, accel=(float("inf"), 0, 0)), capi.TS_ERR_INVALID),
 ])
 def test_viscous_configs_outside_the_built_forms_rejected(kw, code):
-    """the viscous path and the body forces are FP64, 1-D and 2-D; bad coefficients are refused"""
+    """the viscous path and the body forces are FP64; bad coefficients are refused"""
     lib = capi.load()
     cfg = capi.make_config(**kw)
     h = C.c_void_p()

@@ -149,6 +149,56 @@ def test_poiseuille_centre_velocity_on_gpu():
     assert abs(u[11:13].mean() / uc - 1) < 0.01
 
 
+def shear_3d(g):
+    def f(x, y, z):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z)
+        u = 0.3 * np.sin(np.pi * y) * np.sin(np.pi * z)
+        v = -0.2 * np.sin(2 * np.pi * x) + 0.1 * np.cos(np.pi * z)
+        w = 0.15 * np.cos(2 * np.pi * (x + y))
+        p = 1 + 0.1 * np.cos(2 * np.pi * (x - y + z))
+        return util.prim2cons(rho, u, v, w, p, g)
+    return f
+
+
+@pytest.mark.parametrize("recon,lim", [(capi.RECON_PWC, 0), (capi.RECON_LINEAR_LIM, capi.LIMITER_BJ)])
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("stages", [1, 3])
+def test_3d_viscous_matrix(recon, lim, flux, stages):
+    """walls of every kind on the y and z sides, periodic x (RunPoiseuille3D's layout, tests.h:1719)"""
+    cfg = capi.make_config(ndims=3, nx=37, ny=21, nz=17, lx=1.0, ly=0.6, lz=0.5, periodic=(1, 0, 0), cfl=0.45,
+                           riemann=flux, recon=recon, limiter=lim, rk_stages=stages, bcs=MIXED_BCS,
+                           side_marker=(0, 0, 0, 1, 2, 4), viscous=True, viscosity=0.004,
+                           sigma=(1.0, 0.0, 0.2), accel=(0.0, 0.0, -0.5))
+    U = util.fill_state(cfg, shear_3d(cfg.gamma))
+    check(*run_pair(cfg, U, 12))
+
+
+@pytest.mark.parametrize("shape,periodic", [((62, 5, 13), (0, 0, 0)), ((8, 9, 7), (1, 1, 1)),
+                                            ((31, 17, 7), (0, 1, 0))])
+def test_3d_viscous_edge_sizes(shape, periodic):
+    nx, ny, nz = shape
+    cfg = capi.make_config(ndims=3, nx=nx, ny=ny, nz=nz, lz=0.7, periodic=periodic, cfl=0.4,
+                           riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM, rk_stages=3, bcs=MIXED_BCS,
+                           side_marker=(1, 3, 0, 2, 0, 1), viscous=True, viscosity=0.01, sigma=(0.0, 0.5, 0.0))
+    U = util.fill_state(cfg, shear_3d(cfg.gamma))
+    check(*run_pair(cfg, U, 10))
+
+
+def test_3d_viscous_residual_norms():
+    cfg = capi.make_config(ndims=3, nx=30, ny=12, nz=10, periodic=(1, 0, 0), riemann=capi.RIEMANN_ROE,
+                           recon=capi.RECON_LINEAR_LIM, bcs=MIXED_BCS, side_marker=(0, 0, 0, 1, 2, 0),
+                           viscous=True, viscosity=0.01, sigma=(0.3, 0.0, 0.0))
+    U = util.fill_state(cfg, shear_3d(cfg.gamma))
+    gpu = capi.Solver(cfg)
+    gpu.set_state(U)
+    gpu.enable_residual_norms(True)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U)
+    for k in (1, 4):
+        gi, ri = gpu.step(k), ref.step(k)
+        np.testing.assert_allclose(np.array(gi.residual_norm[:5]), np.array(ri.residual_norm[:5]), rtol=1e-12)
+
+
 def test_viscous_kernel_timing_kind():
     cfg = capi.make_config(ndims=2, nx=64, ny=64, periodic=(1, 1, 0), rk_stages=3, recon=capi.RECON_LINEAR_LIM,
                            viscous=True, viscosity=0.01)

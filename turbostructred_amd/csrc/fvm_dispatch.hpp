@@ -21,7 +21,7 @@ struct KernelSet {
     BatchLaunchFn bf1 = nullptr, bf2 = nullptr, bs3 = nullptr;  // fused-dt SSP-RK3, batched
     OccupancyFn bocc = nullptr;
     OccupancyFn socc = nullptr;  // resident workgroups per CU of the single-solve kernels
-    // FP64 instances with viscous terms and body forces (1-D, 2-D): the pre-pass (velocity
+    // FP64 instances with viscous terms and body forces: the pre-pass (velocity
     // gradients and viscous face fluxes of the stage input), the dt pass, the three stages and
     // the residual norms
     LaunchFn vpre = nullptr, vdt = nullptr, vs1 = nullptr, vs2 = nullptr, vs3 = nullptr, vrn = nullptr;

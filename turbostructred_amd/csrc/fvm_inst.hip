@@ -2,9 +2,7 @@
 // per pair with -DTS_DIM=<1|2|3> -DTS_FLUX=<0 Godunov|1 Roe|3 HLLC> so that the kernel families
 // build in parallel.
 #include "fvm_dispatch.hpp"
-#if TS_DIM != 3
 #include "fvm_viscous.hpp"
-#endif
 #if TS_DIM == 3 && __has_include("fvm_kernels3d.hpp")
 #include "fvm_kernels3d.hpp"
 #define TS_HAVE_3D 1
@@ -75,7 +73,6 @@ static int batched_occupancy() {
 }
 #endif
 
-#if TS_DIM != 3
 // viscous pre-pass: gradients of the interior cells, then the faces of every axis
 static void launch_vis_prepass(const KArgs& a, dim3, dim3, hipStream_t s) {
     const unsigned nb = unsigned((a.plane + 255) / 256);
@@ -86,9 +83,12 @@ static void launch_vis_prepass(const KArgs& a, dim3, dim3, hipStream_t s) {
 }
 template <int RC, int LM, int MODE>
 static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
+#if TS_DIM == 3
+    hipLaunchKernelGGL((fvm_stage3d<RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
+#else
     hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
-}
 #endif
+}
 
 template <int RC, int LM>
 static void fill(int precision, KernelSet* ks) {
@@ -107,13 +107,13 @@ static void fill(int precision, KernelSet* ks) {
         ks->bf2 = launch_batched<RC, LM, MODE_F2, double, double, double>;
         ks->bs3 = launch_batched<RC, LM, MODE_S3, double, double, double>;
         ks->bocc = batched_occupancy<RC, LM, double, double>;
+#endif
         ks->vpre = launch_vis_prepass;
         ks->vdt = launch_vis<RC, LM, MODE_DT>;
         ks->vs1 = launch_vis<RC, LM, MODE_S1>;
         ks->vs2 = launch_vis<RC, LM, MODE_S2>;
         ks->vs3 = launch_vis<RC, LM, MODE_S3>;
         ks->vrn = launch_vis<RC, LM, MODE_RN>;
-#endif
     } else {
         ks->dt = launch<RC, LM, MODE_DT, float, double, double>;
         ks->s1e = launch<RC, LM, MODE_S1, float, double, double>;

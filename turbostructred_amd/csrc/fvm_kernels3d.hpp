@@ -9,14 +9,16 @@
 // first barrier -- two barriers per row.  Each z-face flux is evaluated once, by the wavefront
 // above it.  The residual of a cell is accumulated x, then y,
 // then z (ERK:1232,1312,1392) and its spectral radius x-, x+, y-, y+, z-, z+ (ERK:1225-1248,
-// 1305-1328, 1385-1408).
+// 1305-1328, 1385-1408).  VIS instances add the viscous face fluxes of the pre-pass, the
+// viscous spectral-radius term after each plus face (the z-face part travels through LDS with
+// the z eigenvalue) and the body-force sources.
 #pragma once
 
 #include "fvm_kernels.hpp"
 
 namespace tsd {
 
-template <int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
+template <int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0>
 __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     using MT = ModeTraits<MODE>;
     StepCtl* ctl = a.ctl;
@@ -24,6 +26,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     __shared__ R lds_z[ZW][NV][64];  // z+ face state of every plane
     __shared__ R lds_h[ZW][NV][64];  // flux through the z-face below every plane
     __shared__ R lds_l[ZW][64];      // eigenvalue of that face
+    __shared__ double lds_vz[VIS ? ZW : 1][64];  // its viscous spectral-radius term before / V
     __shared__ double red[ZW];
     if constexpr (MT::sr) {
         if (!ctl->active) {
@@ -116,6 +119,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     R dfx[NV], dhz[NV];     // x and z parts of the residual of the pending row
     double srp = 0.0;       // x terms + bottom y term of the pending row's spectral radius
     R lzb = R(0), lzt = R(0);  // z-face eigenvalues of the pending row
+    double vzt = 0.0;          // viscous spectral-radius term (times V) of its top z-face
     R lam_gb = R(0);
     load_cell(y0 - 2, k, Um);
     load_cell(y0 - 1, k, U0);
@@ -162,6 +166,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         if (out_plane && prev_out) {
             lzb = lamzc;
             lzt = lds_l[w + 1][lane];
+            if constexpr (VIS) vzt = lds_vz[w + 1][lane];
             if constexpr (MT::flux) {
 #pragma unroll
                 for (int v = 0; v < NV; v++) dhz[v] = (lds_h[w + 1][v][lane] - Hc[v]) * Szc;
@@ -199,6 +204,23 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             }
             if constexpr (MT::flux) face_flux<FLUX, 2, R>(gamma, eps, ZL, ZR, H, lamz);
             else lamz = face_eigen<FLUX, 2, R>(gamma, ZL, ZR);
+            if constexpr (VIS && MT::flux) {
+                if (a.v.viscous) {
+                    const int kf = k < 0 ? 0 : (k > nz ? nz : k);
+                    const unsigned fi = (unsigned(kf) * unsigned(ny) + unsigned(rcs)) * unsigned(nx) + unsigned(xs);
+#pragma unroll
+                    for (int v = 1; v < NV; v++) H[v] -= R(ld_plane(a.v.fv[2], a.v.nface[2], v - 1, fi));
+                }
+            }
+            if constexpr (VIS && MT::sr) {
+                // max(4/(3 rho_f), gamma/rho_f) mu Sz^2; the plane below divides by its volume
+                const double ro = __builtin_sqrt(double(ZL[0]) * double(ZR[0]));
+                const double va = 4.0 / (3.0 * ro), vb = g.gamma / ro;
+                double vz = (va < vb) ? vb : va;
+                vz *= a.v.mu / 1.0;
+                vz *= double(Sz) * double(Sz);
+                lds_vz[w][lane] = vz;
+            }
             if constexpr (MT::flux) {
 #pragma unroll
                 for (int v = 0; v < NV; v++) lds_h[w][v][lane] = H[v];
@@ -229,14 +251,27 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             R G[NV], lam;
             if constexpr (MT::flux) face_flux<FLUX, 1, R>(gamma, eps, GL, GR, G, lam);
             else lam = face_eigen<FLUX, 1, R>(gamma, GL, GR);
+            if constexpr (VIS && MT::flux) {
+                if (a.v.viscous) {
+                    const unsigned fi = (unsigned(ks) * unsigned(ny + 1) + unsigned(rc)) * unsigned(nx) + unsigned(xs);
+#pragma unroll
+                    for (int v = 1; v < NV; v++) G[v] -= R(ld_plane(a.v.fv[1], a.v.nface[1], v - 1, fi));
+                }
+            }
             if constexpr (MT::sr) {
                 if (t >= 2) {
                     double sr;
                     {
 #pragma clang fp contract(off)
                         sr = srp + double(Sy * lam);
+                    }
+                    if constexpr (VIS)
+                        sr = sr + viscous_sr(double(GL[0]), double(GR[0]), g.gamma, a.v.mu, double(Sy), hx * g.h[1][rc] * hz);
+                    {
+#pragma clang fp contract(off)
                         sr = sr + double(Sz * lzb);
                         sr = sr + double(Sz * lzt);
+                        if constexpr (VIS) sr = sr + vzt / (hx * g.h[1][rc] * hz);
                     }
                     if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc] * hz, sr));
                 }
@@ -250,9 +285,13 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                     const int rf = rc - 1;
                     const double ivol = ihx * g.ih[1][rf + 1] * ihz;
                     const unsigned off = (unsigned(ks) * unsigned(ny) + unsigned(rf)) * unsigned(nx) + unsigned(xs);
+                    R resv[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) resv[v] = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
+                    if constexpr (VIS) add_sources(a.v, Um, hx * g.h[1][rf + 1] * hz, resv);
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
-                        R res = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
+                        R res = resv[v];
                         if constexpr (MODE == MODE_RN) {
                             if (out_lane) rn[v] += hx * g.h[1][rf + 1] * hz * __builtin_fabs(double(res));
                         } else if constexpr (MODE == MODE_F1) {
@@ -283,11 +322,26 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                 R F[NV], lamx;
                 if constexpr (MT::flux) face_flux<FLUX, 0, R>(gamma, eps, UL, UR, F, lamx);
                 else lamx = face_eigen<FLUX, 0, R>(gamma, UL, UR);
+                if constexpr (VIS && MT::flux) {
+                    if (a.v.viscous) {
+                        const unsigned fi = (unsigned(ks) * unsigned(ny) + unsigned(rcs)) * unsigned(nx + 1) +
+                                            unsigned(xg < 0 ? 0 : (xg > nx ? nx : xg));
+#pragma unroll
+                        for (int v = 1; v < NV; v++) F[v] -= R(ld_plane(a.v.fv[0], a.v.nface[0], v - 1, fi));
+                    }
+                }
                 if constexpr (MT::sr) {
                     R lam_r = dpp_from_right(lamx);
                     {
 #pragma clang fp contract(off)
                         srp = double(Sx * lamx) + double(Sx * lam_r);
+                    }
+                    if constexpr (VIS) {
+                        const double rl = dpp_from_right(double(UL[0])), rr = dpp_from_right(double(UR[0]));
+                        srp = srp + viscous_sr(rl, rr, g.gamma, a.v.mu, double(Sx), hx * hy * hz);
+                    }
+                    {
+#pragma clang fp contract(off)
                         srp = srp + double(Sy * lam_gb);
                     }
                     if (ghost_x_lane)  // right ghost column (ERK:1225)

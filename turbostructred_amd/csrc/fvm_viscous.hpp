@@ -1,4 +1,4 @@
-// fvm_viscous.hpp -- viscous pre-pass of one stage (1-D and 2-D): velocity gradients of the
+// fvm_viscous.hpp -- viscous pre-pass of one stage: velocity gradients of the
 // interior cells, then the viscous flux of every face, written for the stage kernel to subtract
 // from its convective fluxes.
 //

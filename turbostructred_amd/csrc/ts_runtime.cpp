@@ -91,7 +91,6 @@ int validate(const ts_config* c, std::string& err) {
                 return TS_ERR_INVALID;
             }
         if (c->precision != TS_FP64) { err = "viscous terms and body forces run in FP64 only"; return TS_ERR_UNSUPPORTED; }
-        if (c->ndims > 2) { err = "viscous terms and body forces are built for 1-D and 2-D grids"; return TS_ERR_UNSUPPORTED; }
     }
     for (int a = 0; a < c->ndims; a++) {
         if (c->periodic[a]) continue;

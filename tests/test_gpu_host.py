@@ -215,3 +215,51 @@ def test_history_and_sensors(tmp_path):
                 b = (p[i + 1] - p[i - 1]) / (2 * h)
                 want = xc[i] - b / (2 * a)
             assert abs(float(x) - want) <= 1e-5, (name, ln, want)
+
+
+POIS = os.path.join(ROOT, "turbostructred_amd", "host", "poiseuille_driver")
+
+
+@pytest.mark.parametrize("init,rk", [(1, 1), (2, 3)])
+def test_poiseuille_driver_matches_oracle_and_centre_velocity(tmp_path, init, rk):
+    """RunSingleExperiment of PoiseuilleChanel.h as a C++ program over the host class: compressed
+    y-axis, wall markers, viscous fluxes and Sigma from KernelConfiguration.  Started 10% above
+    (or below) the exact profile, it relaxes to u_c = sigma Ly^2 / (8 mu); the state matches the
+    oracle run on the program's own grid."""
+    if not os.path.exists(POIS):
+        raise RuntimeError(f"{POIS} is missing: build it (make -C turbostructred_amd/csrc)")
+    nx, ny, mu, sigma, pin, t_end, q = 4, 24, 0.1, 2.0, 100.0, 2.5, 1.02
+    out = str(tmp_path / "pois.bin")
+    args = [POIS, str(nx), str(ny), repr(mu), repr(sigma), repr(pin), repr(t_end), repr(q), str(init), str(rk), out]
+    print("driver:", " ".join(args[1:-1]), flush=True)
+    subprocess.run(args, check=True, timeout=120)
+    raw = open(out, "rb").read()
+    nxa, nya, iters, _ = np.frombuffer(raw[:16], dtype=np.int32)
+    f = np.frombuffer(raw[16:], dtype=np.float64)
+    t, dt, uc = f[:3]
+    o = 3
+    X, HX = f[o:o + nxa], f[o + nxa:o + 2 * nxa]
+    o += 2 * nxa
+    Y, HY = f[o:o + nya], f[o + nya:o + 2 * nya]
+    o += 2 * nya
+    Ud = f[o:].reshape(1, nya, nxa, 5)
+    assert t >= t_end and uc == sigma * 0.25 / (8 * mu)
+    cfg = capi.make_config(ndims=2, nx=nx, ny=ny, lx=1.0, ly=0.5, periodic=(1, 0, 0), cfl=0.45, eps=0.05,
+                           riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM, rk_stages=rk,
+                           bcs=[dict(type=capi.BC_WALL)], viscous=True, viscosity=mu, sigma=(sigma, 0, 0),
+                           coord=[X.copy(), Y.copy(), None], h=[HX.copy(), HY.copy(), None])
+    scale = {0: 1.0, 1: 1.1, 2: 0.9}[init]
+    yy = 1.0 - np.abs(2 * Y / 0.5 - 1.0)
+    u = scale * uc * yy * (2.0 - yy)
+    U0 = np.zeros((1, nya, nxa, 5))
+    U0[..., 0] = 1.0
+    U0[..., 1] = u[None, :, None]
+    U0[..., 4] = (pin / 0.4 + 0.5 * u * u)[None, :, None]
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state(U0)
+    info = ref.step(10_000_000, t_end)
+    assert info.iteration == iters and abs(info.time - t) <= 1e-12 * t
+    a, b = capi.interior(cfg, Ud.ravel()), capi.interior(cfg, ref.get_state())
+    assert np.all(util.normwise_rel(a, b) <= 1e-10)
+    mid = Ud[0, nya // 2 - 1:nya // 2 + 1, nxa // 2]
+    assert abs(np.mean(mid[:, 1] / mid[:, 0]) / uc - 1) < 0.01

@@ -87,7 +87,7 @@ struct BlockNode {
     explicit BlockNode(double p) : pos(p) {}
 };
 
-// src/kernel/KernelConfiguration.h:21-93: the fields of the inviscid explicit path
+// src/kernel/KernelConfiguration.h:21-93: the fields of the explicit path
 struct KernelConfiguration {
     int nDims{1};
     int nX{1}, nY{1}, nZ{1};
@@ -97,7 +97,12 @@ struct KernelConfiguration {
     std::vector<BlockNode> CompressionY{std::vector<BlockNode>(1)};
     std::vector<BlockNode> CompressionZ{std::vector<BlockNode>(1)};
     double Gamma{1.4};
+    double Viscosity{0.0};
     bool IsViscousFlow{false};
+    bool IsExternalForceRequared{false};
+    bool IsUnifromAccelerationRequared{false};
+    Vector Sigma{Vector(0, 0, 0)};                // potential force (pressure gradient)
+    Vector UniformAcceleration{Vector(0, 0, 0)};  // uniform gravity field
     int DummyLayerSize{1};
     MethodConfiguration methodConfiguration;
     int MaxIteration{1000};
@@ -229,7 +234,6 @@ class HipExplicitRungeKuttaFVM {
     // Kernel::Init (kernel.h:178-281) + InitializeMethod: grid, boundary conditions, method,
     // run parameters; creates the device context.
     void Init(KernelConfiguration& config) {
-        if (config.IsViscousFlow) throw std::runtime_error("viscous flow is not part of this library");
         nDims = config.nDims;
         nX = config.nX;
         nY = nDims > 1 ? config.nY : 1;
@@ -268,6 +272,16 @@ class HipExplicitRungeKuttaFVM {
         c.rk_stages = m.RungeKuttaOrder == 3 ? 3 : 1;
         c.precision = m.Precision;
         c.device = config.Device;
+        c.viscous = config.IsViscousFlow;
+        c.viscosity = config.Viscosity;
+        c.external_force = config.IsExternalForceRequared;
+        c.sigma[0] = config.Sigma.x;
+        c.sigma[1] = config.Sigma.y;
+        c.sigma[2] = config.Sigma.z;
+        c.external_accel = config.IsUnifromAccelerationRequared;
+        c.accel[0] = config.UniformAcceleration.x;
+        c.accel[1] = config.UniformAcceleration.y;
+        c.accel[2] = config.UniformAcceleration.z;
 
         // boundary conditions: the reference's markers (any int) become table slots 0..7
         std::map<int, int> slot;

@@ -126,6 +126,51 @@ def tgv_3d(n=256, steps=100):
     return out
 
 
+def viscous_channel(n=2048, steps=40):
+    """Plane channel with viscous fluxes and the potential force Sigma (periodic x, walls at y = 0
+    and y = 0.5, the layout of RunPoiseuille2D), n x n cells, Roe (Eps 0.05) + limited linear,
+    FP64, forward Euler and SSP-RK3, started from the Poiseuille profile at u_c = 0.3."""
+    mu, H, uc = 1e-3, 0.5, 0.3
+    sigma = 8 * mu * uc / H**2
+    out = {"workload": f"viscous channel {n}x{n}, periodic x, walls y, mu {mu}, Sigma ({sigma:.4g}, 0, 0), "
+                       "Roe + limited linear (BJ), CFL 0.45, FP64", "steps": steps}
+    for rk in (1, 3):
+        cfg = capi.make_config(ndims=2, nx=n, ny=n, lx=1.0, ly=H, periodic=(1, 0, 0), cfl=0.45, eps=0.05,
+                               riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM, rk_stages=rk,
+                               bcs=[dict(type=capi.BC_WALL)], viscous=True, viscosity=mu, sigma=(sigma, 0, 0))
+        y = (np.arange(n) + 0.5) * (H / n)
+        u = 4 * uc * y * (H - y) / H**2
+        U = np.zeros(capi.state_shape(cfg))
+        U[..., 0] = 1.0
+        U[1:-1, 1:-1, 1] = u[:, None]
+        U[..., 4] = 1.0 / 0.4
+        U[1:-1, 1:-1, 4] += 0.5 * u[:, None] ** 2
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        s.step(2)
+        t0 = time.perf_counter()
+        info = s.step(steps)
+        el = time.perf_counter() - t0
+        s.enable_kernel_timing(True)
+        s.step(min(steps, 10))
+        kern = {}
+        for kind, name in ((capi.KIND_VISCOUS, "viscous_prepass"), (capi.KIND_DT, "dt_pass"),
+                           (capi.KIND_STAGE1, "stage1"), (capi.KIND_STAGE2, "stage2"), (capi.KIND_STAGE3, "stage3")):
+            ms, cnt = s.kernel_timing(kind)
+            if cnt == 0:
+                continue
+            avg = ms / cnt * 1e-3
+            b = s.algorithmic_bytes_per_cell(kind) * n * n
+            kern[name] = dict(avg_us=avg * 1e6, launches=cnt, achieved_gbs=b / avg / 1e9, frac=b / avg / HBM_PEAK)
+        bps = s.algorithmic_bytes_per_cell(capi.KIND_ALL)
+        rate = n * n * steps / el
+        out["euler" if rk == 1 else "ssp_rk3"] = {
+            "cell_updates_per_s": rate, "ms_per_step": el / steps * 1e3, "algorithmic_bytes_per_cell_step": bps,
+            "step_roofline_frac": rate * bps / HBM_PEAK, "kernels": kern, "final_time": info.time}
+        s.close()
+    return out
+
+
 def cpu_baseline(budget_s=15.0):
     """The FP64 oracle (a CPU port of the reference's step, restated) on this host's cores, on a
     bounded sample of the same workload."""
@@ -299,6 +344,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-viscous", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -350,6 +396,8 @@ def main():
                                 "density_l1_vs_fp64": float(d.mean()), "density_linf_vs_fp64": float(d.max())}
     if not args.no_3d:
         out["tgv_3d"] = tgv_3d()
+    if not args.no_viscous:
+        out["viscous_channel"] = viscous_channel()
     if not args.no_batch:
         out["batched"] = batched_tubes()
     if not args.no_serve:

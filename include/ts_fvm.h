@@ -178,7 +178,8 @@ int ts_enable_kernel_timing(ts_ctx* ctx, int enable);
 int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* launches);
 
 /* Device bytes held by the context and the algorithmic HBM bytes per cell of each kernel
-   kind (compulsory state traffic: one state array = 5*sizeof(real) per cell). */
+   kind of ts_get_kernel_timing except 5 (compulsory traffic: one state array = 5*sizeof(real)
+   per cell; with viscous fluxes, 3*ndims velocity gradients and 4 values per face). */
 int ts_memory_footprint(const ts_ctx* ctx, size_t* device_bytes);
 int ts_algorithmic_bytes_per_cell(const ts_ctx* ctx, int kind, double* bytes);
 

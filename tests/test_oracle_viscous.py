@@ -8,6 +8,9 @@ fluxes subtracted from the convective ones at ERK:1221-1222); the viscous spectr
 (ERK:1239-1248); the force and acceleration sources (kernel.h:941-989); and Poiseuille flow,
 whose centre velocity sigma H^2 / (8 mu) is the reference's own check (PoiseuilleChanel.h:44-47).
 """
+import ctypes as C
+import os
+
 import numpy as np
 import pytest
 
@@ -57,6 +60,48 @@ def test_lsq_gradient_matches_svd_least_squares(nd, m):
         g = po.lsq_gradient(nd, pt, val, pts, vals)
         np.testing.assert_allclose(g[:nd], ref, rtol=1e-11, atol=1e-12)
         assert np.all(g[nd:] == 0)
+
+
+REF_EIGEN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                         "libref_lsq_eigen.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_EIGEN), reason="oracle/_ref not built (needs the reference's Eigen)")
+@pytest.mark.parametrize("nd,m", [(1, 2), (2, 4), (3, 6)])
+def test_lsq_gradient_matches_the_reference_eigen_svd(nd, m):
+    """the reference's own solver (Eigen 3.2.5 JacobiSVD, GradientComputer.h:42-43) on wall
+    T-stencils with random geometry and data: the normal-equations solution agrees to ~1e-14"""
+    lib = C.CDLL(REF_EIGEN)
+    D = C.POINTER(C.c_double)
+    lib.ref_lsq_eigen.argtypes = [C.c_int, D, C.c_double, D, D, C.c_int, D]
+    rng = np.random.default_rng(100 + nd)
+    worst = 0.0
+    for _ in range(200):
+        h = rng.uniform(0.2, 2.0, size=3) * 10.0 ** rng.integers(-4, 1)
+        c = rng.normal(size=3)
+        face = c.copy()
+        face[0] -= 0.5 * h[0]
+        pts = [c]
+        for a in range(1, nd):
+            for sgn in (1, -1):
+                q = c.copy()
+                q[a] += sgn * h[a] * rng.uniform(0.8, 1.25)
+                pts.append(q)
+        gh = c.copy()
+        gh[0] -= h[0] * rng.uniform(0.8, 1.25)
+        pts.append(gh)
+        pts = np.array(pts)
+        assert len(pts) == m
+        vals = rng.normal(size=m)
+        val = rng.normal()
+        g = po.lsq_gradient(nd, face, val, pts, vals)
+        ref = np.empty(3)
+        dp = lambda a: a.ctypes.data_as(D)
+        pf = np.ascontiguousarray(pts)
+        assert lib.ref_lsq_eigen(nd, dp(face), val, dp(pf), dp(vals), m, dp(ref)) == 0
+        scale = np.abs(ref).max()
+        worst = max(worst, np.abs(g - ref).max() / scale)
+    assert worst < 1e-12, worst
 
 
 def test_lsq_gradient_exact_on_linear_data():

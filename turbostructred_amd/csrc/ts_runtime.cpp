@@ -329,7 +329,7 @@ int ts_memory_footprint(const ts_ctx* c, size_t* bytes) {
 }
 
 int ts_algorithmic_bytes_per_cell(const ts_ctx* c, int kind, double* bytes) {
-    if (!c || !bytes || kind < 0 || kind > 4) return TS_ERR_INVALID;
+    if (!c || !bytes || kind < 0 || kind > 6 || kind == 5) return TS_ERR_INVALID;
     // one state array = 5 * sizeof(real) per cell; halo re-reads are not counted
     const double d = 40.0;
     const double s = c->cfg.precision == TS_FP32_RESIDUAL ? 20.0 : 40.0;
@@ -348,8 +348,17 @@ int ts_algorithmic_bytes_per_cell(const ts_ctx* c, int kind, double* bytes) {
         b[2] = s + d + s;    // read U^1, U^n, write U^2
         b[3] = s + d + d;    // read U^2, U^n, write U^{n+1}
     }
-    b[4] = b[0] + b[1] + b[2] + b[3];
-    *bytes = b[kind];
+    // viscous fluxes: each stage also reads 4 face values per axis; the pre-pass writes 3 nd
+    // gradients per cell (reading the state), then reads state and gradients and writes the faces
+    double pre = 0.0;
+    if (c->vis && c->cfg.viscous) {
+        const double nd = c->nd;
+        for (int k = 1; k <= 3; k++)
+            if (b[k] > 0) b[k] += 32.0 * nd;
+        pre = (d + 24.0 * nd) + (d + 24.0 * nd + 32.0 * nd);
+    }
+    b[4] = b[0] + b[1] + b[2] + b[3] + pre * (c->cfg.rk_stages == 1 ? 1 : 3);
+    *bytes = kind == 6 ? pre : b[kind];
     return TS_OK;
 }
 

@@ -455,16 +455,31 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         // ---- march along y ----
         const int y0 = yb * a.ty;
         const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
-        R Um[NV], U0[NV], Up[NV];
-        R Pn[NV];               // prefetched stencil row (consumed as Up one iteration later)
+        R Pn[NV];               // prefetched stencil row (enters the window one iteration later)
         double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
         R ytop[NV];             // +y face state of the previously reconstructed row
-        R gbot[NV];             // y-flux through the bottom face of the pending row
-        R dfx[NV];              // x part of the residual of the pending row
+        // The three-row window (rows rc-1, rc, rc+1) and the values carried to the next row live
+        // in LDS, one set per wavefront; no other lane reads them, so no barrier is needed.  The
+        // window rotates by slot index instead of register moves, and out of the register file
+        // these 25 doubles per lane let three wavefronts share a SIMD.
+        __shared__ R ring[4][3][NV][64];   // [wavefront][slot][variable][lane]
+        __shared__ R carry[4][2][NV][64];  // y-flux through the pending row's bottom face, x residual
+        const int wl = (threadIdx.x >> 6) & 3;
+        R (&gbot)[NV][64] = carry[wl][0];
+        R (&dfx)[NV][64] = carry[wl][1];
+        int sm = 0, s0 = 1, sp = 2;  // slots of rows rc-1, rc, rc+1
         double srp = 0.0;       // spectral radius of the pending row without its top face
         R lam_gb = R(0);
-        load_row(y0 - 2, Um);
-        load_row(y0 - 1, U0);
+        {
+            R t0[NV], t1[NV];
+            load_row(y0 - 2, t0);
+            load_row(y0 - 1, t1);
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                ring[wl][sm][v][lane] = t0[v];
+                ring[wl][s0][v][lane] = t1[v];
+            }
+        }
         load_row(y0, Pn);
 #pragma unroll
         for (int v = 0; v < NV; v++) Nc[v] = 0.0;
@@ -472,7 +487,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         for (int t = 0; t < nit; t++) {
             const int rc = y0 - 1 + t;  // row being reconstructed
 #pragma unroll
-            for (int v = 0; v < NV; v++) Up[v] = Pn[v];
+            for (int v = 0; v < NV; v++) ring[wl][sp][v][lane] = Pn[v];
             if (t + 1 < nit) load_row(rc + 2, Pn);
             if constexpr (MT::use_un) {
                 // U^n of the row finalized below, loaded now: the reconstruction and the y-face
@@ -492,14 +507,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
             geo.rl[1] = R(g.rl[1][rcs]);
             geo.rr[1] = R(g.rr[1][rcs]);
             R nb[4][NV], face[4][NV];
+            {
+                R U0[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                nb[0][v] = dpp_from_left(U0[v]);
-                nb[1][v] = dpp_from_right(U0[v]);
-                nb[2][v] = Um[v];
-                nb[3][v] = Up[v];
+                for (int v = 0; v < NV; v++) {
+                    U0[v] = ring[wl][s0][v][lane];
+                    nb[0][v] = dpp_from_left(U0[v]);
+                    nb[1][v] = dpp_from_right(U0[v]);
+                    nb[2][v] = ring[wl][sm][v][lane];
+                    nb[3][v] = ring[wl][sp][v][lane];
+                }
+                reconstruct<2, RECON, LIM, R, FLUX != FX_ROE>(U0, nb, geo, face);
             }
-            reconstruct<2, RECON, LIM, R, FLUX != FX_ROE>(U0, nb, geo, face);
             if (t >= 1) {
                 // face between rows rc-1 and rc
                 R GL[NV], GR[NV];
@@ -537,16 +556,19 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 }
                 if constexpr (MT::flux) {
                     if (t >= 2) {
-                        // finalize row rc-1 = the row held in Um
+                        // finalize row rc-1 = the row in slot sm
                         const int rf = rc - 1;
                         const double ivol = ihx * g.ih[1][rf + 1];
-                        R res[NV];
+                        R res[NV], Um[NV];
 #pragma unroll
-                        for (int v = 0; v < NV; v++) res[v] = dfx[v] - (G[v] - gbot[v]) * Sy;
+                        for (int v = 0; v < NV; v++) {
+                            res[v] = dfx[v][lane] - (G[v] - gbot[v][lane]) * Sy;
+                            Um[v] = ring[wl][sm][v][lane];
+                        }
                         finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
                     }
 #pragma unroll
-                    for (int v = 0; v < NV; v++) gbot[v] = G[v];
+                    for (int v = 0; v < NV; v++) gbot[v][lane] = G[v];
                 }
                 if (rc < yend) {
                     // x-faces of row rc: each lane evaluates its minus face
@@ -592,17 +614,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
                             R Fr = dpp_from_right(F[v]);
-                            dfx[v] = -((Fr - F[v]) * Sx);
+                            dfx[v][lane] = -((Fr - F[v]) * Sx);
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                ytop[v] = face[3][v];
-                Um[v] = U0[v];
-                U0[v] = Up[v];
-            }
+            for (int v = 0; v < NV; v++) ytop[v] = face[3][v];
+            const int st = sm;
+            sm = s0;
+            s0 = sp;
+            sp = st;
         }
     }
 

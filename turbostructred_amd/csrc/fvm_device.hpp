@@ -31,7 +31,7 @@ struct BCDev {
 // ghost fill (normal -e_axis on both sides, kernel.h:743,760) and the boundary-face states
 // (normal +e_axis, ExplicitRungeKuttaFVM.h:1202,1208) share this one function.
 template <typename R>
-__device__ inline void bc_image(const BCDev& bc, R gamma, int axis, const R (&u)[NV], R (&o)[NV]) {
+__device__ __forceinline__ void bc_image(const BCDev& bc, R gamma, int axis, const R (&u)[NV], R (&o)[NV]) {
     switch (bc.type) {
     case BC_SYMMETRY:
         // U - 2 (n.m) n: only the normal momentum changes sign

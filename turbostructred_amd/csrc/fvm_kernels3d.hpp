@@ -95,13 +95,15 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
 #pragma unroll
             for (int v = 0; v < NV; v++) u[v] = R(ld_plane(uin, P, v, off));
         }
-        int axis = -1, side = 0, fidx = 0;
-        if (xside != 0) { axis = 0; side = xside < 0 ? 0 : 1; fidx = kks * ny + rs; }
-        else if (yside != 0) { axis = 1; side = yside < 0 ? 2 : 3; fidx = kks * nx + xs; }
-        else if (zside != 0) { axis = 2; side = zside < 0 ? 4 : 5; fidx = rs * nx + xs; }
-        if (axis >= 0) {
-            R t[NV];
-            bc_image(g.bc[bc_marker(g, side, fidx)], gamma, axis, u, t);
+        // one call per axis, so that the axis is a constant inside bc_image (a run-time axis
+        // would index the state arrays dynamically and put them in scratch memory)
+        R t[NV];
+        bool img = true;
+        if (xside != 0) bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, kks * ny + rs)], gamma, 0, u, t);
+        else if (yside != 0) bc_image(g.bc[bc_marker(g, yside < 0 ? 2 : 3, kks * nx + xs)], gamma, 1, u, t);
+        else if (zside != 0) bc_image(g.bc[bc_marker(g, zside < 0 ? 4 : 5, rs * nx + xs)], gamma, 2, u, t);
+        else img = false;
+        if (img) {
 #pragma unroll
             for (int v = 0; v < NV; v++) u[v] = t[v];
         }
@@ -115,8 +117,12 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     R Zx[NV];               // centre row of the z-neighbour no wavefront holds (edge wavefronts)
     double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
     R ytop[NV];             // +y face state of the previous row
-    R gbot[NV];             // y-flux through the bottom face of the pending row
-    R dfx[NV], dhz[NV];     // x and z parts of the residual of the pending row
+    // carried to the next row in LDS, per wavefront and lane (no barrier needed): the y-flux
+    // through the pending row's bottom face and the x and z parts of its residual
+    __shared__ R carry[ZW][3][NV][64];
+    R (&gbot)[NV][64] = carry[w][0];
+    R (&dfx)[NV][64] = carry[w][1];
+    R (&dhz)[NV][64] = carry[w][2];
     double srp = 0.0;       // x terms + bottom y term of the pending row's spectral radius
     R lzb = R(0), lzt = R(0);  // z-face eigenvalues of the pending row
     double vzt = 0.0;          // viscous spectral-radius term (times V) of its top z-face
@@ -125,12 +131,11 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     load_cell(y0 - 1, k, U0);
     load_cell(y0, k, Pn);
 #pragma unroll
-    for (int v = 0; v < NV; v++) { Nc[v] = 0.0; dfx[v] = dhz[v] = gbot[v] = R(0); }
+    for (int v = 0; v < NV; v++) { Nc[v] = 0.0; dfx[v][lane] = dhz[v][lane] = gbot[v][lane] = R(0); }
     const bool edge = (w == 0) || (w == ZW - 1);
     const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
-    R Hc[NV], lamzc = R(0), Szc = R(0);  // z-flux below this plane for the previous row
-#pragma unroll
-    for (int v = 0; v < NV; v++) Hc[v] = R(0);
+    // the z-flux below this plane for the previous row is still in lds_h[w] when it is read
+    R lamzc = R(0), Szc = R(0);
     bool prev_out = false;
     const int nit = yend - y0 + 2;
     for (int t = 0; t < nit; t++) {
@@ -169,7 +174,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             if constexpr (VIS) vzt = lds_vz[w + 1][lane];
             if constexpr (MT::flux) {
 #pragma unroll
-                for (int v = 0; v < NV; v++) dhz[v] = (lds_h[w + 1][v][lane] - Hc[v]) * Szc;
+                for (int v = 0; v < NV; v++) dhz[v][lane] = (lds_h[w + 1][v][lane] - lds_h[w][v][lane]) * Szc;
             }
         }
         R face[6][NV];
@@ -227,8 +232,6 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             }
             lds_l[w][lane] = lamz;
         }
-#pragma unroll
-        for (int v = 0; v < NV; v++) Hc[v] = H[v];
         lamzc = lamz;
         Szc = Sz;
         prev_out = row_out;
@@ -287,7 +290,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                     const unsigned off = (unsigned(ks) * unsigned(ny) + unsigned(rf)) * unsigned(nx) + unsigned(xs);
                     R resv[NV];
 #pragma unroll
-                    for (int v = 0; v < NV; v++) resv[v] = (dfx[v] - (G[v] - gbot[v]) * Sy) - dhz[v];
+                    for (int v = 0; v < NV; v++) resv[v] = (dfx[v][lane] - (G[v] - gbot[v][lane]) * Sy) - dhz[v][lane];
                     if constexpr (VIS) add_sources(a.v, Um, hx * g.h[1][rf + 1] * hz, resv);
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
@@ -304,7 +307,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                     }
                 }
 #pragma unroll
-                for (int v = 0; v < NV; v++) gbot[v] = G[v];
+                for (int v = 0; v < NV; v++) gbot[v][lane] = G[v];
             }
             if (row_out) {
                 // x-faces of row rc
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         R Fr = dpp_from_right(F[v]);
-                        dfx[v] = -((Fr - F[v]) * Sx);
+                        dfx[v][lane] = -((Fr - F[v]) * Sx);
                     }
                 }
             }

@@ -77,9 +77,13 @@ static int batched_occupancy() {
 static void launch_vis_prepass(const KArgs& a, dim3, dim3, hipStream_t s) {
     const unsigned nb = unsigned((a.plane + 255) / 256);
     hipLaunchKernelGGL(vis_gradients<TS_DIM>, dim3(nb), dim3(256), 0, s, a);
-    unsigned nf = 0;
-    for (int ax = 0; ax < TS_DIM; ax++) nf = a.v.nface[ax] > nf ? a.v.nface[ax] : nf;
-    hipLaunchKernelGGL(vis_face_flux<TS_DIM>, dim3((nf + 255) / 256, TS_DIM), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((vis_face_flux<TS_DIM, 0>), dim3((a.v.nface[0] + 255) / 256), dim3(256), 0, s, a);
+#if TS_DIM > 1
+    hipLaunchKernelGGL((vis_face_flux<TS_DIM, 1>), dim3((a.v.nface[1] + 255) / 256), dim3(256), 0, s, a);
+#endif
+#if TS_DIM > 2
+    hipLaunchKernelGGL((vis_face_flux<TS_DIM, 2>), dim3((a.v.nface[2] + 255) / 256), dim3(256), 0, s, a);
+#endif
 }
 template <int RC, int LM, int MODE>
 static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {

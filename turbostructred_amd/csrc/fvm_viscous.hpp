@@ -5,7 +5,7 @@
 //   vis_gradients   ComputeGradientNonUniformStructured (kernel.h:918-938): central differences
 //                   (u(i+1) - u(i-1)) / (x(i+1) - x(i-1)) per axis, one thread per cell, ghost
 //                   neighbours from the periodic wrap or the boundary image.
-//   vis_face_flux   ComputeViscousFluxes (ERK:668-874), one thread per face (blockIdx.y = axis):
+//   vis_face_flux   ComputeViscousFluxes (ERK:668-874), one thread per face, one launch per axis:
 //                   normal derivatives from the two cell values, tangential ones from the
 //                   distance-weighted cell gradients.  A dummy cell's gradient is computed where it
 //                   is needed, by the one face that uses it: the periodic partner's gradient
@@ -80,11 +80,12 @@ __global__ void __launch_bounds__(256) vis_gradients(KArgs a) {
 }
 
 // least squares through the normal equations (the same loops as the CPU restatement)
-template <int DIM>
-__device__ inline void vis_lsq(const double (&pt)[3], double value, const double (*pts)[3], const double* vals,
-                               int m, double (&grad)[3]) {
+template <int DIM, int NP>
+__device__ inline void vis_lsq(const double (&pt)[3], double value, const double (&pts)[NP][3], const double (&vals)[NP],
+                               double (&grad)[3]) {
     double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, b[3] = {0, 0, 0};
-    for (int q = 0; q < m; q++) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
         double r[3];
 #pragma unroll
         for (int e = 0; e < DIM; e++) r[e] = pt[e] - pts[q][e];
@@ -128,8 +129,8 @@ __device__ inline void vis_lsq(const double (&pt)[3], double value, const double
 
 // velocity gradient G[d][e] of a cell with interior indices I, of which index I[ax] may be -1 or
 // n (a dummy cell across the faces normal to ax)
-template <int DIM>
-__device__ inline void vis_cell_grad(const KArgs& a, const int (&I)[3], int ax, double (&G)[3][3]) {
+template <int DIM, int ax>
+__device__ inline void vis_cell_grad(const KArgs& a, const int (&I)[3], double (&G)[3][3]) {
     const GridArgs& g = a.g;
     const int n = g.n[ax];
     const bool ghost = I[ax] < 0 || I[ax] >= n;
@@ -148,39 +149,30 @@ __device__ inline void vis_cell_grad(const KArgs& a, const int (&I)[3], int ax, 
     for (int e = 0; e < 3; e++) fc[e] = e < DIM ? a.v.x[e][B[e] + 1] : 0.0;
     const double hb = g.h[ax][B[ax] + 1];
     fc[ax] = side ? a.v.x[ax][B[ax] + 1] + 0.5 * hb : a.v.x[ax][B[ax] + 1] - 0.5 * hb;
-    // stencil: the boundary cell, its transverse neighbours, the dummy cell
-    int S[6][3];
-    int m = 0;
-    auto push = [&](int di, int dj, int dk) {
-        S[m][0] = B[0] + di;
-        S[m][1] = B[1] + dj;
-        S[m][2] = B[2] + dk;
-        m++;
-    };
-    push(0, 0, 0);
-    if (ax == 0) {
-        if (DIM > 1) { push(0, 1, 0); push(0, -1, 0); }
-        if (DIM > 2) { push(0, 0, 1); push(0, 0, -1); }
-    } else if (ax == 1) {
-        push(1, 0, 0); push(-1, 0, 0);
-        if (DIM > 2) { push(0, 0, 1); push(0, 0, -1); }
-    } else {
-        push(1, 0, 0); push(-1, 0, 0); push(0, 1, 0); push(0, -1, 0);
+    // stencil: the boundary cell (slot 0), +-1 along the first transverse axis (1, 2), +-1 along
+    // the second (3, 4), the dummy cell (last) -- 2 DIM points for every axis, in the order of
+    // Stencil::InternalBorderStencil, with constant slots so that the arrays stay in registers
+    constexpr int NP = 2 * DIM;
+    constexpr int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    int S[NP][3];
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        const int e = (q == 1 || q == 2) ? t1 : ((q == 3 || q == 4) ? t2 : -1);
+        const int d = (q == 1 || q == 3) ? 1 : -1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) S[q][c] = (q == NP - 1) ? I[c] : B[c] + (c == e ? d : 0);
     }
-    S[m][0] = I[0];
-    S[m][1] = I[1];
-    S[m][2] = I[2];
-    m++;
-    double pts[6][3], vel[6][3];
+    double pts[NP][3], vel[NP][3];
     double uin[NV], udum[NV];
-    for (int q = 0; q < m; q++) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
         double u[NV];
         vis_load<DIM>(a, S[q][0], S[q][1], S[q][2], u);
         if (q == 0) {
 #pragma unroll
             for (int v = 0; v < NV; v++) uin[v] = u[v];
         }
-        if (q == m - 1) {
+        if (q == NP - 1) {
 #pragma unroll
             for (int v = 0; v < NV; v++) udum[v] = u[v];
         }
@@ -194,19 +186,19 @@ __device__ inline void vis_cell_grad(const KArgs& a, const int (&I)[3], int ax, 
     for (int v = 0; v < NV; v++) Vf[v] = 0.5 * (uin[v] + udum[v]);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double vals[6];
-        for (int q = 0; q < m; q++) vals[q] = vel[q][d];
+        double vals[NP];
+#pragma unroll
+        for (int q = 0; q < NP; q++) vals[q] = vel[q][d];
         double gf[3];
-        vis_lsq<DIM>(fc, Vf[1 + d] / Vf[0], pts, vals, m, gf);
+        vis_lsq<DIM, NP>(fc, Vf[1 + d] / Vf[0], pts, vals, gf);
 #pragma unroll
         for (int e = 0; e < 3; e++) G[d][e] = 2.0 * gf[e] - G[d][e];
     }
 }
 
-template <int DIM>
+template <int DIM, int ax>
 __global__ void __launch_bounds__(256) vis_face_flux(KArgs a) {
     if (!vis_gate(a)) return;
-    const int ax = blockIdx.y;
     const unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= a.v.nface[ax]) return;
     const GridArgs& g = a.g;
@@ -217,8 +209,8 @@ __global__ void __launch_bounds__(256) vis_face_flux(KArgs a) {
     double UL[NV], UR[NV], GL[3][3], GR[3][3];
     vis_load<DIM>(a, L[0], L[1], L[2], UL);
     vis_load<DIM>(a, R[0], R[1], R[2], UR);
-    vis_cell_grad<DIM>(a, L, ax, GL);
-    vis_cell_grad<DIM>(a, R, ax, GR);
+    vis_cell_grad<DIM, ax>(a, L, GL);
+    vis_cell_grad<DIM, ax>(a, R, GR);
     const double* X = a.v.x[ax];
     const double* H = g.h[ax];
     const int ir = R[ax] + 1;  // index of the right cell in the ghost-inclusive arrays
@@ -240,7 +232,7 @@ __global__ void __launch_bounds__(256) vis_face_flux(KArgs a) {
     double t[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const int lo = d < ax ? d : ax, hi = d < ax ? ax : d;
+        const int lo = d < ax ? d : ax, hi = d < ax ? ax : d;  // constants after unrolling
         t[d] = (d == ax) ? tau_d + 2 * mu * D[ax][ax] : mu * (D[lo][hi] + D[hi][lo]);
     }
     double* F = a.v.fv[ax];

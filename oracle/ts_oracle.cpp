@@ -19,7 +19,12 @@
  *   boundary images    per-type closed forms (BCGeneral.h:186-268)
  *   whole step         Sod invariants (mass/energy conservation, momentum growth 0.9*dt),
  *                      first-order convergence to the exact solution, direction invariance
- *                      x/y/z, periodic conservation, diagonal symmetry
+ *                      x/y/z, periodic conservation, diagonal symmetry, second-order
+ *                      convergence on the smooth Liska-Wendroff solution
+ *                      (ExactEulerSolution.h:30-44; orders 2.5 and 2.3 at 16/32/64)
+ *   viscous terms      least squares vs numpy SVD and the reference's Eigen JacobiSVD, wall
+ *                      dummy gradients, Couette and stress-divergence closed forms, viscous SR,
+ *                      force/acceleration sources, Poiseuille centre velocity
  *   residual norms     first Sod step: norm = 2 h |F*| of the diaphragm face (kernel.h:326)
  * The exact star state is pinned against the mathematics (f(p*) = 0 to round-off) and Toro's
  * 5-digit table; the reference's own p* comes from ALGLIB's tolerance-limited LM

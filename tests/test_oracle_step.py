@@ -214,3 +214,42 @@ def test_residual_norm_first_sod_step():
     assert abs(info.residual_norm[0] - 2 * h * abs(F[0])) <= 1e-15 * abs(F[0])
     # energy: F_L and F_R of the resting states carry no energy flux either
     assert abs(info.residual_norm[4] - 2 * h * abs(F[4])) <= 1e-14 * abs(F[4])
+
+
+@pytest.mark.parametrize("riemann", [capi.RIEMANN_ROE, capi.RIEMANN_HLLC])
+def test_liska_wendroff_smooth_solution_second_order(riemann):
+    """The smooth 2-D solution of the reference's ExactEulerSolution test
+    (src/Tests/2DTests/ExactEulerSolution.h:30-44, Liska & Wendroff 2002): rho = 1 + 0.2
+    sin(2 pi (x + y - t (u + v))), u = 1, v = -0.5, p = 1, periodic.  Central linear
+    reconstruction + SSP-RK3 must converge at second order in the density L1 error against the
+    exact cell averages (a wrong slope factor or sign, or a first-order stage combination, drops
+    the order to one or below)."""
+    a, u, v, t_end = 0.2, 1.0, -0.5, 0.25
+    errs = []
+    for n in (16, 32, 64):
+        h = 1.0 / n
+        c = (np.arange(n) + 0.5) * h
+        Y, X = np.meshgrid(c, c, indexing="ij")
+        avg = (np.sin(np.pi * h) / (np.pi * h)) ** 2  # cell average of sin over an h x h cell
+
+        def rho_avg(t):
+            return 1.0 + a * avg * np.sin(2 * np.pi * (X + Y - t * (u + v)))
+
+        cfg = capi.make_config(ndims=2, nx=n, ny=n, periodic=(1, 1, 0), cfl=0.4, riemann=riemann,
+                               recon=capi.RECON_LINEAR, rk_stages=3)
+        r0 = rho_avg(0.0)
+        U = np.zeros(capi.state_shape(cfg))
+        I = U[0, 1:-1, 1:-1]
+        I[..., 0] = r0
+        I[..., 1] = r0 * u
+        I[..., 2] = r0 * v
+        I[..., 4] = 1.0 / (G - 1) + 0.5 * r0 * (u * u + v * v)
+        o = po.Oracle(cfg, threads=4)
+        o.set_state(U.ravel())
+        info = o.step(100000, t_max=t_end, next_snapshot=t_end)
+        assert abs(info.time - t_end) < 1e-14
+        rho = o.get_state().reshape(U.shape)[0, 1:-1, 1:-1, 0]
+        errs.append(np.abs(rho - rho_avg(t_end)).mean())
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    print("L1 errors", errs, "orders", orders)
+    assert np.all(orders > 1.8), (errs, orders)

@@ -1,12 +1,13 @@
 // fvm_kernels.hpp -- fused finite-volume stage kernels for gfx950 (1-D and 2-D marching form).
 //
 // One wavefront owns a strip of 64 columns along x (60 output columns plus a halo of two
-// lanes on each side) and marches along y over `ty` output rows, keeping a three-row window of
-// the state in registers and the next row in flight (prefetched one iteration ahead).
-// x-neighbours come from the adjacent lanes with DPP wave shifts, so no LDS is used; every face
-// flux is evaluated once, by the lane on its plus side, and the residual of a cell is
-// accumulated in registers in x-then-y order (ERK:1232,1312) before the stage update is applied
-// and the row is stored with coalesced SoA writes.
+// lanes on each side) and marches along y over `ty` output rows.  In 2-D it keeps a three-row
+// window of the state, and the values carried from one row to the next, in its own LDS slots
+// (no other lane reads them, so there are no barriers), with the next row in flight in
+// registers (prefetched one iteration ahead).  x-neighbours come from the adjacent lanes with
+// DPP wave shifts; every face flux is evaluated once, by the lane on its plus side, and the
+// residual of a cell is accumulated in x-then-y order (ERK:1232,1312) before the stage update
+// is applied and the row is stored with coalesced SoA writes.
 //
 // Modes
 //   MODE_DT  spectral radius of every cell from the face eigenvalues, min(V/SR) over the grid

@@ -3,7 +3,8 @@
 // A workgroup is ZW = 8 wavefronts, one per z-plane k = k0-1+w; wavefronts 1..6 produce output
 // (ZOUT = 6 planes per workgroup) and the two edge wavefronts only supply reconstructions (each
 // loads the one extra centre row its outer z-neighbour needs).  Inside a plane the scheme is the
-// 2-D one: 64 lanes along x (60 output columns) and a register window marching along y.
+// 2-D one: 64 lanes along x (60 output columns) and a three-row window marching along y, with
+// the values carried from row to row in per-wavefront LDS slots.
 // z-neighbours travel through LDS once per row: the centre row of the state, then the z+ face
 // state of each plane, then the z-face flux, which is read one row later after the next row's
 // first barrier -- two barriers per row.  Each z-face flux is evaluated once, by the wavefront

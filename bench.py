@@ -171,6 +171,32 @@ def viscous_channel(n=2048, steps=40):
     return out
 
 
+def sod_c1_config():
+    """The reference's own CPU case: Sod, 1000 cells, piecewise constant + Roe (Eps 0), forward
+    Euler, CFL 0.4, Natural boundaries, run until Time >= 0.2 (tests.h:13-80)."""
+    return capi.make_config(ndims=1, nx=1000, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC,
+                            bcs=[dict(type=capi.BC_NATURAL)])
+
+
+def sod_c1():
+    """Time to solution of the Sod case on the GPU, state resident (the run is launch-bound:
+    1000 cells per step, two launches per step replayed from hipGraphs)."""
+    import util
+    cfg = sod_c1_config()
+    U = util.sod(cfg)
+    s = capi.Solver(cfg)
+    s.set_state(U)
+    s.step(5)  # warm-up: graph capture and first launches
+    s.set_state(U)
+    s.set_time(0.0, 0)
+    t0 = time.perf_counter()
+    info = s.step(1 << 20, 0.2)
+    el = time.perf_counter() - t0
+    return {"workload": "Sod 1-D 1000 cells, PWC + Roe, forward Euler, CFL 0.4, to t = 0.2",
+            "steps": info.steps_done, "final_time": info.time, "seconds": el,
+            "steps_per_s": info.steps_done / el, "cell_updates_per_s": 1000 * info.steps_done / el}
+
+
 def cpu_baseline(budget_s=15.0):
     """The FP64 oracle (a CPU port of the reference's step, restated) on this host's cores, on a
     bounded sample of the same workload."""
@@ -189,9 +215,20 @@ def cpu_baseline(budget_s=15.0):
     t0 = time.perf_counter()
     o.step(k)
     el = time.perf_counter() - t0
-    return dict(value=n * n * k / el, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"same problem at {n}x{n}, {k} SSP-RK3 steps (+1 warm-up), FP64 oracle, "
-                       f"{threads} std::threads over grid lines")
+    out = dict(value=n * n * k / el, unit="cell-updates/s", cores=threads, kind="port",
+               sample=f"same problem at {n}x{n}, {k} SSP-RK3 steps (+1 warm-up), FP64 oracle, "
+                      f"{threads} std::threads over grid lines")
+    # the Sod case run whole on one core (the reference's configuration runs single-rank)
+    import util
+    c1 = sod_c1_config()
+    o1 = pyoracle.Oracle(c1, threads=1)
+    o1.set_state(util.sod(c1))
+    t0 = time.perf_counter()
+    i1 = o1.step(1 << 20, 0.2)
+    el1 = time.perf_counter() - t0
+    out["sod_c1"] = {"seconds": el1, "steps": i1.steps_done, "cores": 1,
+                     "cell_updates_per_s": 1000 * i1.steps_done / el1}
+    return out
 
 
 def batched_tubes(count=4096, sample=48, seed=2024):
@@ -345,6 +382,7 @@ def main():
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-viscous", action="store_true")
+    ap.add_argument("--no-sod", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -398,6 +436,8 @@ def main():
         out["tgv_3d"] = tgv_3d()
     if not args.no_viscous:
         out["viscous_channel"] = viscous_channel()
+    if not args.no_sod:
+        out["sod_c1"] = sod_c1()
     if not args.no_batch:
         out["batched"] = batched_tubes()
     if not args.no_serve:

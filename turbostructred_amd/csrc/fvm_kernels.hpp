@@ -184,9 +184,40 @@ __device__ inline double dt_candidate(double vol, double sr) {
     return (q > 0.0) ? q : 0.0;
 }
 
-// Block min of V/SR, one atomic per block, and the last block to finish turns the global
-// minimum into dt: dt = min(V/SR) * CFL (ERK:1430), clamped to the next snapshot (ERK:1433),
-// then advances Time / Iteration and evaluates the stop rule of Kernel::Run (kernel.h:1834-1846).
+// dt = min(V/SR) * CFL (ERK:1430) from the reduced minimum in the control block, clamped to the
+// next snapshot (ERK:1433); advances Time / Iteration, logs dt, evaluates the stop rule of
+// Kernel::Run (kernel.h:1834-1846) and re-arms the reduction cells.  Run by the last
+// contributor of a launch.
+__device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
+    __threadfence();
+    double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
+    double dtn = mn * cfl;
+    double t0 = ctl->time;
+    if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
+    if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
+        ctl->error = 1;
+        ctl->go = 0;
+        ctl->active = 0;
+    } else {
+        double t1 = t0 + dtn;
+        ctl->dt = dtn;
+        ctl->time = t1;
+        long long it = ctl->iter;
+        ctl->iter = it + 1;
+        int sd = ctl->steps_done + 1;
+        ctl->steps_done = sd;
+        long long li = it - ctl->dt_log_base;
+        if (ctl->dt_log && li >= 0 && li < ctl->dt_log_cap) ctl->dt_log[li] = dtn;
+        ctl->go = 1;
+        bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) || (sd >= ctl->max_steps);
+        ctl->active = stop ? 0 : 1;
+    }
+    atomicExch(&ctl->min_bits, ~0ull);
+    atomicExch(&ctl->blocks_done, 0u);
+}
+
+// Block min of V/SR (wave shuffle, LDS across waves), one atomic min and one counter increment
+// per block; the last block to finish finalizes dt.
 __device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block_min, double* red,
                                           int lane, int wave, int nwaves) {
     double m = block_min;
@@ -198,35 +229,20 @@ __device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block
         for (int w = 1; w < nwaves; w++) m = __builtin_fmin(m, red[w]);
         atomicMin(&ctl->min_bits, dbits(m));
         __threadfence();
-        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
-            double dtn = mn * cfl;
-            double t0 = ctl->time;
-            if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
-            if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
-                ctl->error = 1;
-                ctl->go = 0;
-                ctl->active = 0;
-            } else {
-                double t1 = t0 + dtn;
-                ctl->dt = dtn;
-                ctl->time = t1;
-                long long it = ctl->iter;
-                ctl->iter = it + 1;
-                int sd = ctl->steps_done + 1;
-                ctl->steps_done = sd;
-                long long li = it - ctl->dt_log_base;
-                if (ctl->dt_log && li >= 0 && li < ctl->dt_log_cap) ctl->dt_log[li] = dtn;
-                ctl->go = 1;
-                bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
-                            (sd >= ctl->max_steps);
-                ctl->active = stop ? 0 : 1;
-            }
-            atomicExch(&ctl->min_bits, ~0ull);
-            atomicExch(&ctl->blocks_done, 0u);
-        }
+        if (atomicAdd(&ctl->blocks_done, 1u) == gridDim.x - 1) dt_finalize(ctl, cfl);
+    }
+}
+
+// Wavefront min of V/SR, one atomic min and one counter increment per wavefront with no
+// barrier; the last of `count` contributors finalizes dt (the batched kernel, whose wavefronts
+// take tiles of different problems).
+__device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, int lane, int count) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+    if (lane == 0) {
+        atomicMin(&ctl->min_bits, dbits(m));
+        __threadfence();
+        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize(ctl, cfl);
     }
 }
 
@@ -659,6 +675,8 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     double block_min = __builtin_inf();
     double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, VIS>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
+    // block-level dt reduction: one returning counter atomic per block (one per wavefront made
+    // F1 40% slower at 2048^2 -- those atomics serialise at one address)
     if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
     if constexpr (MODE == MODE_RN) {
         __shared__ double rred[4 * NV];
@@ -674,40 +692,6 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
 // neighbouring tiles of a problem share an L2.  The dt of each problem comes from a wavefront-level
 // minimum and the last tile of that problem.  A cell's arithmetic depends only on its own problem,
 // so a result is the same alone or in a mix.
-__device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, int lane, int ntiles) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
-    if (lane == 0) {
-        atomicMin(&ctl->min_bits, dbits(m));
-        __threadfence();
-        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
-        if (prev == unsigned(ntiles - 1)) {
-            __threadfence();
-            double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
-            double dtn = mn * cfl;
-            double t0 = ctl->time;
-            if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
-            if (!(dtn > 0.0) || !(dtn < __builtin_inf())) {
-                ctl->error = 1;
-                ctl->go = 0;
-                ctl->active = 0;
-            } else {
-                double t1 = t0 + dtn;
-                ctl->dt = dtn;
-                ctl->time = t1;
-                ctl->iter = ctl->iter + 1;
-                int sd = ctl->steps_done + 1;
-                ctl->steps_done = sd;
-                ctl->go = 1;
-                bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) ||
-                            (sd >= ctl->max_steps);
-                ctl->active = stop ? 0 : 1;
-            }
-            atomicExch(&ctl->min_bits, ~0ull);
-            atomicExch(&ctl->blocks_done, 0u);
-        }
-    }
-}
 
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
 __global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict__ descs,

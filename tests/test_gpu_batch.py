@@ -212,3 +212,18 @@ def test_server_thread_equals_driven_batch():
     for t1, t2 in zip(tk, tk2):
         assert got[t2][1].steps_done == ref[t1][1].steps_done == 60
         np.testing.assert_array_equal(got[t2][0], ref[t1][0])
+
+
+@pytest.mark.parametrize("kw", [dict(viscous=True, viscosity=0.01), dict(sigma=(1.0, 0.0, 0.0)),
+                                dict(accel=(0.0, -1.0, 0.0)), dict(viscosity=0.01)])
+def test_batch_refuses_viscous_and_force_terms(kw):
+    """the batched engine has no viscous pre-pass or source terms: such a solve is refused at
+    submission with TS_ERR_UNSUPPORTED instead of being run without them"""
+    cfg, U = quad(64, 48, 0)
+    cfg2 = capi.make_config(ndims=2, nx=64, ny=48, cfl=0.45, riemann=capi.RIEMANN_HLLC,
+                            recon=capi.RECON_LINEAR_LIM, rk_stages=3, **kw)
+    b = capi.Batch(hbm_budget_bytes=1 << 26)
+    with pytest.raises(capi.TSError) as e:
+        b.submit(cfg2, U, 10)
+    assert e.value.code == capi.TS_ERR_UNSUPPORTED
+    b.close()

@@ -11,8 +11,8 @@
 //
 // Modes
 //   MODE_DT  spectral radius of every cell from the face eigenvalues, min(V/SR) over the grid
-//            (wave shuffle, LDS across waves, one atomic per block); the last block to finish
-//            turns it into dt (ERK:1427-1437).  Used before forward Euler.
+//            (wave shuffle, one atomic min per wavefront); a one-wavefront kernel launched
+//            after it turns the minimum into dt (ERK:1427-1437).  Used before forward Euler.
 //   MODE_S1  U1 = U + dt L(U)          forward Euler (kernel.h:319-331) / SSP stage 1
 //   MODE_S2  U2 = 3/4 Un + 1/4 (U1 + dt L(U1))
 //   MODE_S3  U  = 1/3 Un + 2/3 (U2 + dt L(U2))   (written in place over Un: Un is read
@@ -216,8 +216,18 @@ __device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
     atomicExch(&ctl->blocks_done, 0u);
 }
 
+// The finalize as a kernel of its own, launched right after a 2-D dt pass or fused stage 1 whose
+// wavefronts only post their minima (non-returning atomics): nothing in the stage kernel waits
+// for a barrier or a returned counter.  One wavefront; skipped when no step is running.  (A
+// template only so that the translation units including this header share one definition.)
+template <int = 0>
+__global__ void __launch_bounds__(64) dt_finalize_kernel(StepCtl* ctl, double cfl) {
+    if (threadIdx.x != 0 || !ctl->active) return;
+    dt_finalize(ctl, cfl);
+}
+
 // Block min of V/SR (wave shuffle, LDS across waves), one atomic min and one counter increment
-// per block; the last block to finish finalizes dt.
+// per block; the last block to finish finalizes dt (the 3-D kernel).
 __device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block_min, double* red,
                                           int lane, int wave, int nwaves) {
     double m = block_min;
@@ -653,7 +663,6 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     static_assert(DIM == 1 || DIM == 2, "3-D uses its own kernel");
     using MT = ModeTraits<MODE>;
     StepCtl* ctl = a.ctl;
-    __shared__ double red[4];
     if constexpr (MT::sr) {
         if (!ctl->active) {
             if (blockIdx.x == 0 && threadIdx.x == 0) ctl->go = 0;
@@ -669,15 +678,20 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
-    if constexpr (!MT::sr && MODE != MODE_RN) {
-        if (tile >= a.ntiles) return;  // no block-level synchronisation in these modes
+    if constexpr (MODE != MODE_RN) {
+        if (tile >= a.ntiles) return;  // only the norms reduce across the block
     }
     double block_min = __builtin_inf();
     double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, VIS>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
-    // block-level dt reduction: one returning counter atomic per block (one per wavefront made
-    // F1 40% slower at 2048^2 -- those atomics serialise at one address)
-    if constexpr (MT::sr) dt_reduce_finalize(ctl, a.g.cfl, block_min, red, lane, wave, nwaves);
+    if constexpr (MT::sr) {
+        // post the wavefront's minimum; dt_finalize_kernel turns it into dt after this launch.
+        // (A returned counter per wavefront to find the last one made F1 40% slower at 2048^2:
+        // those atomics serialise at one address.)
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) block_min = __builtin_fmin(block_min, __shfl_xor(block_min, o));
+        if (lane == 0) atomicMin(&ctl->min_bits, dbits(block_min));
+    }
     if constexpr (MODE == MODE_RN) {
         __shared__ double rred[4 * NV];
         rn_reduce_finalize(ctl, a.rn_part, rn, rred, lane, wave, nwaves);

@@ -28,8 +28,8 @@ static void launch(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
     hipLaunchKernelGGL((fvm_stage3d<RC, LM, TS_FLUX, MODE, R, TIn, TOut>), g, b, 0, s, a);
 #else
     hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, R, TIn, TOut>), g, b, 0, s, a);
-    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
 #endif
+    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
 }
 
 // resident workgroups per CU of the single-solve stage kernels (the minimum over the kernels of
@@ -92,8 +92,8 @@ static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
     hipLaunchKernelGGL((fvm_stage3d<RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
 #else
     hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
-    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
 #endif
+    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
 }
 
 template <int RC, int LM>

@@ -216,31 +216,14 @@ __device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
     atomicExch(&ctl->blocks_done, 0u);
 }
 
-// The finalize as a kernel of its own, launched right after a 2-D dt pass or fused stage 1 whose
-// wavefronts only post their minima (non-returning atomics): nothing in the stage kernel waits
-// for a barrier or a returned counter.  One wavefront; skipped when no step is running.  (A
+// The finalize as a kernel of its own, launched right after a dt pass or fused stage 1 whose
+// wavefronts (3-D: workgroups) only post their minima with non-returning atomics: nothing in the
+// stage kernel waits for a returned counter.  One wavefront; skipped when no step is running.  (A
 // template only so that the translation units including this header share one definition.)
 template <int = 0>
 __global__ void __launch_bounds__(64) dt_finalize_kernel(StepCtl* ctl, double cfl) {
     if (threadIdx.x != 0 || !ctl->active) return;
     dt_finalize(ctl, cfl);
-}
-
-// Block min of V/SR (wave shuffle, LDS across waves), one atomic min and one counter increment
-// per block; the last block to finish finalizes dt (the 3-D kernel).
-__device__ inline void dt_reduce_finalize(StepCtl* ctl, double cfl, double block_min, double* red,
-                                          int lane, int wave, int nwaves) {
-    double m = block_min;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
-    if (lane == 0) red[wave] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < nwaves; w++) m = __builtin_fmin(m, red[w]);
-        atomicMin(&ctl->min_bits, dbits(m));
-        __threadfence();
-        if (atomicAdd(&ctl->blocks_done, 1u) == gridDim.x - 1) dt_finalize(ctl, cfl);
-    }
 }
 
 // Wavefront min of V/SR, one atomic min and one counter increment per wavefront with no

@@ -367,7 +367,18 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
             U0[v] = Up[v];
         }
     }
-    if constexpr (MT::sr) dt_reduce_finalize(ctl, g.cfl, block_min, red, lane, w, ZW);
+    if constexpr (MT::sr) {
+        // block minimum, posted with a non-returning atomic; dt_finalize_kernel follows the launch
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) block_min = __builtin_fmin(block_min, __shfl_xor(block_min, o));
+        if (lane == 0) red[w] = block_min;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = red[0];
+            for (int q = 1; q < ZW; q++) m = __builtin_fmin(m, red[q]);
+            atomicMin(&ctl->min_bits, dbits(m));
+        }
+    }
     if constexpr (MODE == MODE_RN) {
         __shared__ double rred[ZW * NV];
         rn_reduce_finalize(ctl, a.rn_part, rn, rred, lane, w, ZW);

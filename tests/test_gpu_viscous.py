@@ -209,3 +209,29 @@ def test_viscous_kernel_timing_kind():
     s.step(5)
     ms, n = s.kernel_timing(capi.KIND_VISCOUS)
     assert n == 15 and ms > 0
+
+
+def test_2d_viscous_composite_sides():
+    """Per-face markers on three sides with the viscous terms on: the pre-pass's boundary images
+    (velocity gradients, wall least squares, face values) take each face's own condition."""
+    nx, ny = 70, 52
+    xl = (np.arange(ny) % 3).astype(np.uint8)           # Wall / moving wall / symmetry
+    yl = ((np.arange(nx) // 7) % 2 * 3).astype(np.uint8)  # Wall / natural in blocks of 7
+    yr = (1 + (np.arange(nx) % 2)).astype(np.uint8)      # moving wall / symmetry alternating
+    cfg = capi.make_config(ndims=2, nx=nx, ny=ny, ly=0.8, cfl=0.4, riemann=capi.RIEMANN_HLLC,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=3, bcs=MIXED_BCS, side_marker=(0, 4, 0, 0, 0, 0),
+                           face_marker=[xl, None, yl, yr, None, None], viscous=True, viscosity=0.01,
+                           sigma=(0.5, 0.0, 0.0))
+    U = util.fill_state(cfg, shear_2d(cfg.gamma))
+    check(*run_pair(cfg, U, 25))
+
+
+def test_3d_viscous_composite_side():
+    nx, ny, nz = 30, 14, 11
+    zl = ((np.arange(nx * ny) // 5) % 3).astype(np.uint8)  # z-low side, faces numbered j * nx + i
+    xr = ((np.arange(ny * nz) // 3) % 2).astype(np.uint8)  # x-high side, faces numbered k * ny + j
+    cfg = capi.make_config(ndims=3, nx=nx, ny=ny, nz=nz, lz=0.6, cfl=0.4, riemann=capi.RIEMANN_ROE,
+                           recon=capi.RECON_LINEAR_LIM, rk_stages=1, bcs=MIXED_BCS, side_marker=(3, 0, 2, 2, 0, 1),
+                           face_marker=[None, xr, None, None, zl, None], viscous=True, viscosity=0.01)
+    U = util.fill_state(cfg, shear_3d(cfg.gamma))
+    check(*run_pair(cfg, U, 12))

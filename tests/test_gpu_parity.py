@@ -315,3 +315,18 @@ def test_autotune_changes_nothing_but_the_tiling():
     ty, ms = capi.autotune(cfg)
     assert 1 <= ty <= 260 and ms > 0
     np.testing.assert_array_equal(run(), before)
+
+
+@pytest.mark.parametrize("nx,separate", [(157, False), (157, True), (5000, False)])
+@pytest.mark.parametrize("flux", FLUXES)
+def test_1d_euler_launch_forms(nx, separate, flux, monkeypatch):
+    """1-D forward Euler in FP64 runs chunks of steps in one workgroup when the grid has at most
+    64 strips, and one dt pass + one stage launch per step otherwise (or with TS_SEPARATE_DT=1):
+    both against the oracle, with snapshot stops inside a chunk."""
+    if separate:
+        monkeypatch.setenv("TS_SEPARATE_DT", "1")
+    cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=flux, recon=capi.RECON_LINEAR_LIM,
+                           bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL)], side_marker=(0, 1, 0, 0, 0, 0))
+    U = util.fill_state(cfg, smooth_1d(cfg.gamma))
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 300, t_max=1e300, snap=0.3, chunks=3)
+    check(a, b, dg, dr, gi, ri)

@@ -25,6 +25,10 @@ struct KernelSet {
     // gradients and viscous face fluxes of the stage input), the dt pass, the three stages and
     // the residual norms
     LaunchFn vpre = nullptr, vdt = nullptr, vs1 = nullptr, vs2 = nullptr, vs3 = nullptr, vrn = nullptr;
+    // small 1-D forward-Euler solves: a chunk of steps in one workgroup (KArgs.uin = U^n,
+    // uout = the other state buffer); FP64, 1-D units only
+    using ChunkFn = void (*)(const KArgs&, int steps, hipStream_t);
+    ChunkFn euler1d_chunk = nullptr;
 };
 
 // reconstruction variants: 0 PWC, 1 linear, 2 ENO2, 3 limited linear BJ, 4 limited linear Venkat

@@ -96,6 +96,16 @@ static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
     if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
 }
 
+#if TS_DIM == 1
+template <int RC, int LM>
+static void launch_euler1d_chunk(const KArgs& a, int steps, hipStream_t s) {
+    KArgs b = a;  // the other direction of the ping-pong
+    b.uin = a.uout;
+    b.uout = const_cast<void*>(a.uin);
+    hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX>), dim3(1), dim3(64 * WG1D_WAVES), 0, s, a, b, steps);
+}
+#endif
+
 template <int RC, int LM>
 static void fill(int precision, KernelSet* ks) {
     if (precision == 0) {
@@ -120,6 +130,9 @@ static void fill(int precision, KernelSet* ks) {
         ks->vs2 = launch_vis<RC, LM, MODE_S2>;
         ks->vs3 = launch_vis<RC, LM, MODE_S3>;
         ks->vrn = launch_vis<RC, LM, MODE_RN>;
+#if TS_DIM == 1
+        ks->euler1d_chunk = launch_euler1d_chunk<RC, LM>;
+#endif
     } else {
         ks->dt = launch<RC, LM, MODE_DT, float, double, double>;
         ks->s1e = launch<RC, LM, MODE_S1, float, double, double>;

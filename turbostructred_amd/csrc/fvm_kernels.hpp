@@ -681,6 +681,66 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     }
 }
 
+// ---- small 1-D forward-Euler solves: a chunk of steps in one workgroup -------------------------
+// A grid of a few strips is launch-bound (two short kernels per step).  One 512-thread
+// workgroup runs up to `cap` steps of a ts_step call instead: the dt pass over its strips (each
+// wavefront takes strips w, w + 8, ...), the workgroup minimum in LDS, the finalize by one
+// thread, a barrier, the forward-Euler stage into the other state buffer and a barrier before
+// the next dt pass reads it.  The two directions of the state ping-pong come as two kernel
+// arguments (a0: buffer 0 -> 1, a1: 1 -> 0), so that nothing is copied per step.  Every
+// wavefront leaves the loop together: after a step that ends the run (stop rule, error) or
+// after `cap` steps.
+constexpr int WG1D_WAVES = 8;
+
+// one step; false when the loop has to end
+template <int RECON, int LIM, int FLUX>
+__device__ __forceinline__ bool euler1d_step(const KArgs& a, double* red, int* run, int lane, int wave) {
+    StepCtl* ctl = a.ctl;
+    if (threadIdx.x == 0) *run = ctl->active;
+    __syncthreads();
+    if (!*run) {
+        if (threadIdx.x == 0) ctl->go = 0;
+        return false;
+    }
+    double m = __builtin_inf();
+    for (int tile = wave; tile < a.ntiles; tile += WG1D_WAVES) {
+        double mt = __builtin_inf();
+        stage_tile<1, RECON, LIM, FLUX, MODE_DT, double, double, double>(a, tile, lane, mt);
+        m = __builtin_fmin(m, mt);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bmin = red[0];
+        for (int w = 1; w < WG1D_WAVES; w++) bmin = __builtin_fmin(bmin, red[w]);
+        atomicMin(&ctl->min_bits, dbits(bmin));
+        dt_finalize(ctl, a.g.cfl);
+        *run = ctl->go;
+    }
+    __syncthreads();
+    if (!*run) return false;  // non-physical step: the state of the last good step stays in place
+    for (int tile = wave; tile < a.ntiles; tile += WG1D_WAVES) {
+        double unused = 0.0;
+        stage_tile<1, RECON, LIM, FLUX, MODE_S1, double, double, double>(a, tile, lane, unused);
+    }
+    __syncthreads();
+    return true;
+}
+
+template <int RECON, int LIM, int FLUX>
+__global__ void __launch_bounds__(64 * WG1D_WAVES) fvm_euler1d_workgroup(KArgs a0, KArgs a1, int cap) {
+    __shared__ double red[WG1D_WAVES];
+    __shared__ int run;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int it = 0; it < cap; it += 2) {
+        if (!euler1d_step<RECON, LIM, FLUX>(a0, red, &run, lane, wave)) break;
+        if (it + 1 >= cap) break;
+        if (!euler1d_step<RECON, LIM, FLUX>(a1, red, &run, lane, wave)) break;
+    }
+}
+
 // ---- batched kernel: a persistent grid over a table of (problem, tile) pairs -----------------
 // Every problem of the batch keeps its own KArgs (state, geometry, control block).  Wavefront w of
 // the persistent grid takes tiles w, w + W, w + 2W, ... of the table (W = wavefronts in the grid):

@@ -26,6 +26,8 @@ struct ts_ctx {
     // SSP-RK3 with the dt pass fused into stage 1 (dU[1] then holds L(U^n)); off when the
     // environment sets TS_RK3_SEPARATE_DT=1
     bool fused_dt = true;
+    // small 1-D forward-Euler solves run chunks of steps in one workgroup (no launch per step)
+    bool euler1d_wg = false;
 
     // geometry on the device
     std::vector<double*> dbufs;  // everything to free

@@ -213,6 +213,8 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     c->host_stage.resize(c->ncell * 5);
     const char* sep = getenv("TS_RK3_SEPARATE_DT");
     c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
+    c->euler1d_wg = k.rk_stages == 1 && c->nd == 1 && k.precision == TS_FP64 && !c->vis && c->ks.euler1d_chunk &&
+                    c->ntiles <= 64 && !(sep && sep[0] == '1');
     return TS_OK;
 }
 
@@ -523,7 +525,52 @@ int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap) {
     int launched = 0;
     const int start_buf = c->cur;
     int rc = TS_OK;
-    while (launched < max_steps) {
+    const bool workgroup_chunks = c->euler1d_wg && !c->norms;
+    if (workgroup_chunks) {
+        // chunks of steps in one workgroup; the state buffer of the next chunk follows the steps
+        // taken so far
+        constexpr int WG_CHUNK = 4096;
+        KArgs a{};
+        a.g = c->grid;
+        a.plane = c->ncell;
+        a.ctl = c->dctl;
+        a.ty = c->ty;
+        a.nstrips = c->nstrips;
+        a.ntiles = c->ntiles;
+        int done = 0;
+        while (launched < max_steps) {
+            const int chunk = std::min(max_steps - launched, WG_CHUNK);
+            const int in = (start_buf + done) & 1;
+            a.uin = c->dU[in];
+            a.uout = c->dU[in ^ 1];
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (c->timing) {
+                while (c->events.size() < 2) {
+                    hipEvent_t e;
+                    CHK(hipEventCreate(&e));
+                    c->events.push_back(e);
+                }
+                e0 = c->events[0];
+                e1 = c->events[1];
+                CHK(hipEventRecord(e0, c->stream));
+            }
+            c->ks.euler1d_chunk(a, chunk, c->stream);
+            CHK(hipGetLastError());
+            if (c->timing) CHK(hipEventRecord(e1, c->stream));
+            launched += chunk;
+            CHK(hipMemcpyAsync(h, c->dctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+            CHK(hipStreamSynchronize(c->stream));
+            if (c->timing) {
+                float ms = 0;
+                CHK(hipEventElapsedTime(&ms, e0, e1));
+                c->t_ms[1] += ms;
+                c->t_n[1] += 1;
+            }
+            done = h->steps_done;
+            if (!h->active) break;
+        }
+    }
+    while (!workgroup_chunks && launched < max_steps) {
         int chunk = std::min(max_steps - launched, CHUNK);
         // executed steps always form a prefix of the issued ones, so the Euler ping-pong buffer
         // of the final state is (start + steps_done) mod 2 whatever the chunking

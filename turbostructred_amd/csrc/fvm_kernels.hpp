@@ -184,13 +184,10 @@ __device__ inline double dt_candidate(double vol, double sr) {
     return (q > 0.0) ? q : 0.0;
 }
 
-// dt = min(V/SR) * CFL (ERK:1430) from the reduced minimum in the control block, clamped to the
-// next snapshot (ERK:1433); advances Time / Iteration, logs dt, evaluates the stop rule of
-// Kernel::Run (kernel.h:1834-1846) and re-arms the reduction cells.  Run by the last
-// contributor of a launch.
-__device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
-    __threadfence();
-    double mn = bitsd(atomicAdd(&ctl->min_bits, 0ull));
+// dt = min(V/SR) * CFL (ERK:1430) from the grid minimum mn, clamped to the next snapshot
+// (ERK:1433); advances Time / Iteration, logs dt and evaluates the stop rule of Kernel::Run
+// (kernel.h:1834-1846).  One thread.
+__device__ inline void dt_apply(StepCtl* ctl, double cfl, double mn) {
     double dtn = mn * cfl;
     double t0 = ctl->time;
     if (ctl->next_snap > 0 && ctl->next_snap < t0 + dtn) dtn = ctl->next_snap - t0;
@@ -212,6 +209,12 @@ __device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
         bool stop = (t1 >= ctl->t_max) || (ctl->next_snap > 0 && t1 == ctl->next_snap) || (sd >= ctl->max_steps);
         ctl->active = stop ? 0 : 1;
     }
+}
+
+// dt_apply on the minimum the contributors of a launch posted, then re-arms the reduction cells
+__device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
+    __threadfence();
+    dt_apply(ctl, cfl, bitsd(atomicAdd(&ctl->min_bits, 0ull)));
     atomicExch(&ctl->min_bits, ~0ull);
     atomicExch(&ctl->blocks_done, 0u);
 }
@@ -715,8 +718,8 @@ __device__ __forceinline__ bool euler1d_step(const KArgs& a, double* red, int* r
     if (threadIdx.x == 0) {
         double bmin = red[0];
         for (int w = 1; w < WG1D_WAVES; w++) bmin = __builtin_fmin(bmin, red[w]);
-        atomicMin(&ctl->min_bits, dbits(bmin));
-        dt_finalize(ctl, a.g.cfl);
+        // the whole grid is this workgroup: no atomics on the control block
+        dt_apply(ctl, a.g.cfl, bmin);
         *run = ctl->go;
     }
     __syncthreads();

@@ -173,7 +173,9 @@ int ts_get_dt_log(ts_ctx* ctx, double* out, long long capacity, long long* n_out
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch.
    kind: 0 = dt pass, 1 = stage 1 / Euler stage, 2 = stage 2, 3 = stage 3, 4 = kinds 0-3,
-   5 = residual-norm pass, 6 = viscous pre-pass (velocity gradients and viscous face fluxes). */
+   5 = residual-norm pass, 6 = viscous pre-pass (velocity gradients and viscous face fluxes).
+   1-D forward Euler in FP64 on at most 64 strips runs chunks of steps in one workgroup: each
+   chunk is one kind-1 launch (dt passes included) and kind 0 stays empty. */
 int ts_enable_kernel_timing(ts_ctx* ctx, int enable);
 int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* launches);
 

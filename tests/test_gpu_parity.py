@@ -41,6 +41,18 @@ def run_pair(cfg, U, steps, t_max=1e300, snap=0.0, chunks=1):
     return a, b, np.array(dts_gpu), np.array(dts_ref), ginfo, rinfo
 
 
+def dt_pass_launches(cfg, U, steps):
+    """Number of stand-alone dt-pass launches (timing kind 0) that `steps` steps make: one per
+    step in the separate-dt sequences, none in the fused SSP-RK3 form or the one-workgroup 1-D
+    Euler chunks.  Shows which launch form a context took."""
+    s = capi.Solver(cfg)
+    s.set_state(U)
+    s.enable_kernel_timing(True)
+    info = s.step(steps)
+    assert info.steps_done == steps
+    return s.kernel_timing(capi.KIND_DT)[1]
+
+
 def check(a, b, dg, dr, gi, ri, tol=TOL):
     assert len(dg) == len(dr), (len(dg), len(dr))
     np.testing.assert_array_less(np.abs(dg - dr), tol * np.abs(dr) + 1e-300)
@@ -213,10 +225,13 @@ def test_unphysical_state_reports_error():
 @pytest.mark.parametrize("precision", [capi.FP64])
 def test_rk3_separate_dt_pass_path(flux, precision, monkeypatch):
     """SSP-RK3 with the stand-alone dt pass (the fused-dt path is the default)."""
-    monkeypatch.setenv("TS_RK3_SEPARATE_DT", "1")
     cfg = capi.make_config(ndims=2, nx=77, ny=64, cfl=0.45, riemann=flux, recon=capi.RECON_LINEAR_LIM,
                            rk_stages=3, precision=precision)
     U = util.quadrants(cfg)
+    monkeypatch.delenv("TS_SEPARATE_DT", raising=False)
+    assert dt_pass_launches(cfg, U, 3) == 0
+    monkeypatch.setenv("TS_SEPARATE_DT", "1")
+    assert dt_pass_launches(cfg, U, 3) >= 3
     a, b, dg, dr, gi, ri = run_pair(cfg, U, 20)
     check(a, b, dg, dr, gi, ri)
 
@@ -328,5 +343,8 @@ def test_1d_euler_launch_forms(nx, separate, flux, monkeypatch):
     cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=flux, recon=capi.RECON_LINEAR_LIM,
                            bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL)], side_marker=(0, 1, 0, 0, 0, 0))
     U = util.fill_state(cfg, smooth_1d(cfg.gamma))
+    workgroup = nx <= 64 * 60 and not separate
+    n_dt = dt_pass_launches(cfg, U, 3)
+    assert (n_dt == 0) if workgroup else (n_dt >= 3), n_dt
     a, b, dg, dr, gi, ri = run_pair(cfg, U, 300, t_max=1e300, snap=0.3, chunks=3)
     check(a, b, dg, dr, gi, ri)

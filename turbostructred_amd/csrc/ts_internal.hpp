@@ -24,9 +24,10 @@ struct ts_ctx {
     size_t stage_elem = 8;
     int cur = 0;
     // SSP-RK3 with the dt pass fused into stage 1 (dU[1] then holds L(U^n)); off when the
-    // environment sets TS_RK3_SEPARATE_DT=1
+    // environment sets TS_SEPARATE_DT=1
     bool fused_dt = true;
-    // small 1-D forward-Euler solves run chunks of steps in one workgroup (no launch per step)
+    // small 1-D forward-Euler solves run chunks of steps in one workgroup (no launch per step);
+    // also off with TS_SEPARATE_DT=1
     bool euler1d_wg = false;
 
     // geometry on the device

@@ -211,7 +211,7 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->grid_dim = dim3(c->ntiles);
     }
     c->host_stage.resize(c->ncell * 5);
-    const char* sep = getenv("TS_RK3_SEPARATE_DT");
+    const char* sep = getenv("TS_SEPARATE_DT");
     c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
     c->euler1d_wg = k.rk_stages == 1 && c->nd == 1 && k.precision == TS_FP64 && !c->vis && c->ks.euler1d_chunk &&
                     c->ntiles <= 64 && !(sep && sep[0] == '1');

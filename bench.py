@@ -385,7 +385,7 @@ def main():
     ap.add_argument("--no-sod", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+        print(f"bench.py measures one GPU only (the library has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
     prec = capi.FP64 if args.precision == "fp64" else capi.FP32_RESIDUAL

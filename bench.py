@@ -39,10 +39,14 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
     t_set = time.perf_counter() - t0
     if warmup:
         s.step(warmup)
+    # ts_step returns only after the library's stream has drained (it reads the step-control
+    # block back), so the timed region starts and ends with the GPU idle.  torch's synchronize is
+    # added where torch sees the device; its own HIP runtime finds none in a process where the
+    # library's runtime already holds it, and the bracket is then the library's own.
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is only used for the contract's synchronisation bracket
+    except Exception:
         sync = lambda: None
     sync()
     t0 = time.perf_counter()
@@ -354,6 +358,39 @@ def serving(rates=(50, 100, 200, 500, 1000, 2000), requests=48, seed=7):
     return out
 
 
+_COPY_PROBE = """
+import torch
+n = {nbytes}
+x = torch.empty(n // 8, dtype=torch.float64, device="cuda")
+y = torch.empty_like(x)
+x.fill_(1.0)
+for _ in range(3):
+    y.copy_(x)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range({reps}):
+    y.copy_(x)
+e1.record()
+e1.synchronize()
+print(2 * n / (e0.elapsed_time(e1) * 1e-3 / {reps}) / 1e9)
+"""
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Device-to-device copy rate in GB/s (bytes read + bytes written per second) of two 1 GiB
+    buffers, four times the 256 MB Infinity Cache, so the copy runs at the rate HBM sustains; it
+    is printed next to the vendor peak.  torch times the copies with events on its own stream, in
+    a child process: torch brings its own HIP runtime, which finds no device in a process where
+    the library's runtime already holds it.  None if the child fails."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, "-c", _COPY_PROBE.format(nbytes=nbytes, reps=reps)],
+                           capture_output=True, text=True, timeout=300)
+        return float(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
     path = os.path.join(ROOT, "profiles", "c2_2048_fp64_hbm_traffic.md")
@@ -396,6 +433,7 @@ def main():
     dname, d = dom
     traffic = traffic_from_profiles(dname)
     bytes_step = sum(s.get("algorithmic_bytes", 0) for s in kern.values()) / ncell
+    copy_gbs = copy_bandwidth()
     out = {
         "metric": "cell-updates/s & % HBM roofline; served-solve p50/p99 at L1 err<=1e-5",
         "value": value,
@@ -414,7 +452,9 @@ def main():
                    "cells": ncell, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": d["achieved_gbs"], "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
-                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": d["avg_us"]},
+                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": d["avg_us"],
+                     "measured_copy_gbs": copy_gbs,
+                     "frac_of_measured_copy": d["achieved_gbs"] / copy_gbs if copy_gbs else None},
         "step_roofline_frac": value * bytes_step / HBM_PEAK,
         "algorithmic_bytes_per_cell_step": bytes_step,
         "kernels": kern,

@@ -201,6 +201,35 @@ def sod_c1():
             "steps_per_s": info.steps_done / el, "cell_updates_per_s": 1000 * info.steps_done / el}
 
 
+def host_cores():
+    """Logical CPUs of the host, the ones this process may run on, and physical cores (distinct
+    (package, core) pairs of /proc/cpuinfo; None where that file does not say)."""
+    logical = os.cpu_count()
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = logical
+    phys = None
+    try:
+        pairs, pkg, core = set(), None, None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "physical id":
+                pkg = v.strip()
+            elif k == "core id":
+                core = v.strip()
+            elif not k and pkg is not None and core is not None:
+                pairs.add((pkg, core))
+                pkg = core = None
+        if pkg is not None and core is not None:
+            pairs.add((pkg, core))
+        phys = len(pairs) or None
+    except OSError:
+        pass
+    return {"logical_cpus": logical, "usable_cpus": usable, "physical_cores": phys}
+
+
 def cpu_baseline(budget_s=15.0):
     """The FP64 oracle (a CPU port of the reference's step, restated) on this host's cores, on a
     bounded sample of the same workload."""
@@ -221,7 +250,7 @@ def cpu_baseline(budget_s=15.0):
     el = time.perf_counter() - t0
     out = dict(value=n * n * k / el, unit="cell-updates/s", cores=threads, kind="port",
                sample=f"same problem at {n}x{n}, {k} SSP-RK3 steps (+1 warm-up), FP64 oracle, "
-                      f"{threads} std::threads over grid lines")
+                      f"{threads} std::threads over grid lines", host=host_cores())
     # the Sod case run whole on one core (the reference's configuration runs single-rank)
     import util
     c1 = sod_c1_config()

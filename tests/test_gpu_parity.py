@@ -171,6 +171,22 @@ def test_2d_edge_sizes(shape):
     check(a, b, dg, dr, gi, ri)
 
 
+@pytest.mark.parametrize("nx", [2, 3, 59, 60, 61, 121, 3840, 3841])
+@pytest.mark.parametrize("stages", [1, 3])
+def test_1d_edge_sizes(nx, stages):
+    """1-D grids at the smallest size, around the 60-column strip width, and at the largest grid
+    (64 strips) whose forward-Euler steps run in one workgroup and the first size past it."""
+    cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM,
+                           rk_stages=stages, bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL)],
+                           side_marker=(0, 1, 0, 0, 0, 0))
+    U = util.sod(cfg)
+    if stages == 1:
+        n_dt = dt_pass_launches(cfg, U, 3)
+        assert (n_dt == 0) if nx <= 64 * 60 else (n_dt >= 3), n_dt
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 30)
+    check(a, b, dg, dr, gi, ri)
+
+
 def test_snapshot_clamp_and_termination():
     cfg = capi.make_config(ndims=1, nx=300, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_PWC)
     U = util.sod(cfg)

@@ -102,7 +102,7 @@ static void launch_euler1d_chunk(const KArgs& a, int steps, hipStream_t s) {
     KArgs b = a;  // the other direction of the ping-pong
     b.uin = a.uout;
     b.uout = const_cast<void*>(a.uin);
-    hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX>), dim3(1), dim3(64 * WG1D_WAVES), 0, s, a, b, steps);
+    hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX>), dim3(1), dim3(64 * wg1d_waves<TS_FLUX>), 0, s, a, b, steps);
 }
 #endif
 

@@ -332,9 +332,11 @@ __device__ inline void add_sources(const VisArgs& v, const R (&u)[NV], double vo
 // VIS instances (FP64) add the viscous face fluxes of the pre-pass (subtracted from the convective
 // flux before it is differenced, ERK:1221-1222), the viscous spectral-radius term after each
 // plus face's eigenvalue, and the body-force sources after the flux sweeps.
-template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0>
+// DT_ARG instances take the step's dt from `dt_arg` instead of the control block in memory.
+template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0,
+          bool DT_ARG = false>
 __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min,
-                                           double* rn = nullptr) {
+                                           double* rn = nullptr, double dt_arg = 0.0) {
     using MT = ModeTraits<MODE>;
     const GridArgs& g = a.g;
     const StepCtl* ctl = a.ctl;
@@ -352,7 +354,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     const bool ghost_x_lane = tile_ok && xg == nx && lane >= 2 && lane <= 62;
 
     const R gamma = R(g.gamma), eps = R(g.eps);
-    const double dt = MT::sr ? 0.0 : ctl->dt;
+    const double dt = MT::sr ? 0.0 : (DT_ARG ? dt_arg : ctl->dt);
 
     CellGeo<DIM, R> geo;
     const double hx = g.h[0][xs + 1];
@@ -695,20 +697,25 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
 }
 
 // ---- small 1-D forward-Euler solves: a chunk of steps in one workgroup -------------------------
-// A grid of a few strips is launch-bound (two short kernels per step).  One 512-thread
-// workgroup runs up to `cap` steps of a ts_step call instead: the dt pass over its strips (each
-// wavefront takes strips w, w + 8, ...), the workgroup minimum in LDS, the finalize by one
+// A grid of a few strips is launch-bound (two short kernels per step).  One workgroup of W
+// wavefronts runs up to `cap` steps of a ts_step call instead: the dt pass over its strips (each
+// wavefront takes strips w, w + W, ...), the workgroup minimum in LDS, the finalize by one
 // thread, a barrier, the forward-Euler stage into the other state buffer and a barrier before
 // the next dt pass reads it.  The two directions of the state ping-pong come as two kernel
 // arguments (a0: buffer 0 -> 1, a1: 1 -> 0), so that nothing is copied per step.  Every
 // wavefront leaves the loop together: after a step that ends the run (stop rule, error) or
-// after `cap` steps.
-constexpr int WG1D_WAVES = 8;
+// after `cap` steps.  The control block lives in LDS for the chunk (thread 0 copies it in at the
+// start and back at the end), so the per-step bookkeeping -- the stop test, the dt update and
+// the stage's read of dt -- makes no global round trip.
+// W = 16 covers the 17 strips of 1000 cells in two rounds instead of three; the exact solver's
+// registers do not fit the 128 per lane of a 1024-thread workgroup, so Godunov keeps W = 8.
+template <int FLUX>
+constexpr int wg1d_waves = FLUX == FX_GODUNOV ? 8 : 16;
 
 // one step; false when the loop has to end
 template <int RECON, int LIM, int FLUX>
-__device__ __forceinline__ bool euler1d_step(const KArgs& a, double* red, int* run, int lane, int wave) {
-    StepCtl* ctl = a.ctl;
+__device__ __forceinline__ bool euler1d_step(const KArgs& a, StepCtl* ctl, double* red, int* run, int lane,
+                                             int wave) {
     if (threadIdx.x == 0) *run = ctl->active;
     __syncthreads();
     if (!*run) {
@@ -716,7 +723,7 @@ __device__ __forceinline__ bool euler1d_step(const KArgs& a, double* red, int* r
         return false;
     }
     double m = __builtin_inf();
-    for (int tile = wave; tile < a.ntiles; tile += WG1D_WAVES) {
+    for (int tile = wave; tile < a.ntiles; tile += wg1d_waves<FLUX>) {
         double mt = __builtin_inf();
         stage_tile<1, RECON, LIM, FLUX, MODE_DT, double, double, double>(a, tile, lane, mt);
         m = __builtin_fmin(m, mt);
@@ -727,31 +734,35 @@ __device__ __forceinline__ bool euler1d_step(const KArgs& a, double* red, int* r
     __syncthreads();
     if (threadIdx.x == 0) {
         double bmin = red[0];
-        for (int w = 1; w < WG1D_WAVES; w++) bmin = __builtin_fmin(bmin, red[w]);
+        for (int w = 1; w < wg1d_waves<FLUX>; w++) bmin = __builtin_fmin(bmin, red[w]);
         // the whole grid is this workgroup: no atomics on the control block
         dt_apply(ctl, a.g.cfl, bmin);
         *run = ctl->go;
     }
     __syncthreads();
     if (!*run) return false;  // non-physical step: the state of the last good step stays in place
-    for (int tile = wave; tile < a.ntiles; tile += WG1D_WAVES) {
+    const double dt = ctl->dt;
+    for (int tile = wave; tile < a.ntiles; tile += wg1d_waves<FLUX>) {
         double unused = 0.0;
-        stage_tile<1, RECON, LIM, FLUX, MODE_S1, double, double, double>(a, tile, lane, unused);
+        stage_tile<1, RECON, LIM, FLUX, MODE_S1, double, double, double, 0, true>(a, tile, lane, unused, nullptr, dt);
     }
     __syncthreads();
     return true;
 }
 
 template <int RECON, int LIM, int FLUX>
-__global__ void __launch_bounds__(64 * WG1D_WAVES) fvm_euler1d_workgroup(KArgs a0, KArgs a1, int cap) {
-    __shared__ double red[WG1D_WAVES];
+__global__ void __launch_bounds__(64 * wg1d_waves<FLUX>) fvm_euler1d_workgroup(KArgs a0, KArgs a1, int cap) {
+    __shared__ double red[wg1d_waves<FLUX>];
     __shared__ int run;
+    __shared__ StepCtl ctl;  // a0.ctl and a1.ctl are the same block
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) ctl = *a0.ctl;  // read only by thread 0 until the first barrier
     for (int it = 0; it < cap; it += 2) {
-        if (!euler1d_step<RECON, LIM, FLUX>(a0, red, &run, lane, wave)) break;
+        if (!euler1d_step<RECON, LIM, FLUX>(a0, &ctl, red, &run, lane, wave)) break;
         if (it + 1 >= cap) break;
-        if (!euler1d_step<RECON, LIM, FLUX>(a1, red, &run, lane, wave)) break;
+        if (!euler1d_step<RECON, LIM, FLUX>(a1, &ctl, red, &run, lane, wave)) break;
     }
+    if (threadIdx.x == 0) *a0.ctl = ctl;  // the other threads only ever read the LDS copy
 }
 
 // ---- batched kernel: a persistent grid over a table of (problem, tile) pairs -----------------

@@ -171,11 +171,12 @@ def test_2d_edge_sizes(shape):
     check(a, b, dg, dr, gi, ri)
 
 
-@pytest.mark.parametrize("nx", [2, 3, 59, 60, 61, 121, 3840, 3841])
+@pytest.mark.parametrize("nx", [2, 3, 59, 60, 61, 121, 1984, 1985, 3840, 3841])
 @pytest.mark.parametrize("stages", [1, 3])
 def test_1d_edge_sizes(nx, stages):
-    """1-D grids at the smallest size, around the 60-column strip width, and at the largest grid
-    (64 strips) whose forward-Euler steps run in one workgroup and the first size past it."""
+    """1-D grids at the smallest size, around the 60-column strip width, at the largest grid whose
+    one-workgroup forward-Euler chunks keep the state in LDS (1984 cells) and the first past it,
+    and at the largest grid (64 strips) whose steps run in one workgroup and the first past it."""
     cfg = capi.make_config(ndims=1, nx=nx, cfl=0.4, riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM,
                            rk_stages=stages, bcs=[dict(type=capi.BC_NATURAL), dict(type=capi.BC_WALL)],
                            side_marker=(0, 1, 0, 0, 0, 0))

@@ -102,7 +102,12 @@ static void launch_euler1d_chunk(const KArgs& a, int steps, hipStream_t s) {
     KArgs b = a;  // the other direction of the ping-pong
     b.uin = a.uout;
     b.uout = const_cast<void*>(a.uin);
-    hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX>), dim3(1), dim3(64 * wg1d_waves<TS_FLUX>), 0, s, a, b, steps);
+    if (a.g.n[0] <= WG1D_LDS_CELLS)
+        hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX, true>), dim3(1), dim3(64 * wg1d_waves<TS_FLUX>), 0, s,
+                           a, b, steps);
+    else
+        hipLaunchKernelGGL((fvm_euler1d_workgroup<RC, LM, TS_FLUX, false>), dim3(1), dim3(64 * wg1d_waves<TS_FLUX>), 0, s,
+                           a, b, steps);
 }
 #endif
 

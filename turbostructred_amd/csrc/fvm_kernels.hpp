@@ -332,11 +332,14 @@ __device__ inline void add_sources(const VisArgs& v, const R (&u)[NV], double vo
 // VIS instances (FP64) add the viscous face fluxes of the pre-pass (subtracted from the convective
 // flux before it is differenced, ERK:1221-1222), the viscous spectral-radius term after each
 // plus face's eigenvalue, and the body-force sources after the flux sweeps.
-// DT_ARG instances take the step's dt from `dt_arg` instead of the control block in memory.
+// DT_ARG instances take the step's dt from `dt_arg` instead of the control block in memory;
+// LDS_STATE instances read and write the state through `uin_arg` / `uout_arg` with plane stride
+// `plane_arg` (LDS arrays, addressed generically) with plain stores.
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0,
-          bool DT_ARG = false>
+          bool DT_ARG = false, bool LDS_STATE = false>
 __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min,
-                                           double* rn = nullptr, double dt_arg = 0.0) {
+                                           double* rn = nullptr, double dt_arg = 0.0, const void* uin_arg = nullptr,
+                                           void* uout_arg = nullptr, size_t plane_arg = 0) {
     using MT = ModeTraits<MODE>;
     const GridArgs& g = a.g;
     const StepCtl* ctl = a.ctl;
@@ -365,10 +368,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     geo.rr[0] = R(g.rr[0][xs]);
     geo.eps2 = R(g.eps2);
 
-    const TIn* __restrict__ uin = (const TIn*)a.uin;
+    const TIn* __restrict__ uin = LDS_STATE ? (const TIn*)uin_arg : (const TIn*)a.uin;
     const TOut* __restrict__ lin = (const TOut*)a.lin;
-    TOut* uo = (TOut*)a.uout;
-    const size_t P = a.plane;
+    TOut* uo = LDS_STATE ? (TOut*)uout_arg : (TOut*)a.uout;
+    const size_t P = LDS_STATE ? plane_arg : a.plane;
+    auto store = [&](int v, unsigned off, TOut x) {
+        if constexpr (LDS_STATE) st_plane(uo, P, v, off, x);
+        else st_plane_stream(uo, P, v, off, x);
+    };
 
     // one stencil value: the input state at a (possibly ghost) cell of row r
     auto load_row = [&](int r, R (&u)[NV]) {
@@ -408,11 +415,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
             if constexpr (MODE == MODE_RN) {
                 if (out_lane) rn[v] += vol * __builtin_fabs(double(res[v]));
             } else if constexpr (MODE == MODE_F1) {
-                if (out_lane) st_plane_stream(uo, P, v, off, TOut(double(res[v]) * ivol));
+                if (out_lane) store(v, off, TOut(double(res[v]) * ivol));
             } else {
                 double q = double(u[v]) + dt * (double(res[v]) * ivol);
                 if constexpr (MT::use_un) q = MT::ca * un[v] + MT::cb * q;
-                if (out_lane) st_plane_stream(uo, P, v, off, TOut(q));
+                if (out_lane) store(v, off, TOut(q));
             }
         }
     };
@@ -709,13 +716,17 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
 // the stage's read of dt -- makes no global round trip.
 // W = 16 covers the 17 strips of 1000 cells in two rounds instead of three; the exact solver's
 // registers do not fit the 128 per lane of a 1024-thread workgroup, so Godunov keeps W = 8.
+// Grids of at most WG1D_LDS_CELLS cells keep both state buffers in LDS for the chunk (copied in
+// from the input buffer at the start, the final state copied out to the buffer the host expects
+// it in at the end): the dt pass and the stage then make no global round trip at all.
 template <int FLUX>
 constexpr int wg1d_waves = FLUX == FX_GODUNOV ? 8 : 16;
+constexpr int WG1D_LDS_CELLS = 1984;  // 2 buffers x 5 planes x 1984 doubles = 158720 bytes
 
 // one step; false when the loop has to end
-template <int RECON, int LIM, int FLUX>
+template <int RECON, int LIM, int FLUX, bool LDS>
 __device__ __forceinline__ bool euler1d_step(const KArgs& a, StepCtl* ctl, double* red, int* run, int lane,
-                                             int wave) {
+                                             int wave, const double* sin, double* sout) {
     if (threadIdx.x == 0) *run = ctl->active;
     __syncthreads();
     if (!*run) {
@@ -725,7 +736,8 @@ __device__ __forceinline__ bool euler1d_step(const KArgs& a, StepCtl* ctl, doubl
     double m = __builtin_inf();
     for (int tile = wave; tile < a.ntiles; tile += wg1d_waves<FLUX>) {
         double mt = __builtin_inf();
-        stage_tile<1, RECON, LIM, FLUX, MODE_DT, double, double, double>(a, tile, lane, mt);
+        stage_tile<1, RECON, LIM, FLUX, MODE_DT, double, double, double, 0, false, LDS>(
+            a, tile, lane, mt, nullptr, 0.0, sin, sout, WG1D_LDS_CELLS);
         m = __builtin_fmin(m, mt);
     }
 #pragma unroll
@@ -744,23 +756,49 @@ __device__ __forceinline__ bool euler1d_step(const KArgs& a, StepCtl* ctl, doubl
     const double dt = ctl->dt;
     for (int tile = wave; tile < a.ntiles; tile += wg1d_waves<FLUX>) {
         double unused = 0.0;
-        stage_tile<1, RECON, LIM, FLUX, MODE_S1, double, double, double, 0, true>(a, tile, lane, unused, nullptr, dt);
+        stage_tile<1, RECON, LIM, FLUX, MODE_S1, double, double, double, 0, true, LDS>(
+            a, tile, lane, unused, nullptr, dt, sin, sout, WG1D_LDS_CELLS);
     }
     __syncthreads();
     return true;
 }
 
-template <int RECON, int LIM, int FLUX>
+template <int RECON, int LIM, int FLUX, bool LDS>
 __global__ void __launch_bounds__(64 * wg1d_waves<FLUX>) fvm_euler1d_workgroup(KArgs a0, KArgs a1, int cap) {
+    constexpr int NT = 64 * wg1d_waves<FLUX>;
     __shared__ double red[wg1d_waves<FLUX>];
     __shared__ int run;
     __shared__ StepCtl ctl;  // a0.ctl and a1.ctl are the same block
+    __shared__ double st[LDS ? 2 * NV * WG1D_LDS_CELLS : 1];
+    double* b0 = st;
+    double* b1 = st + (LDS ? NV * WG1D_LDS_CELLS : 0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ncell = a0.g.n[0];  // at most WG1D_LDS_CELLS when LDS (checked by the launcher)
     if (threadIdx.x == 0) ctl = *a0.ctl;  // read only by thread 0 until the first barrier
+    if constexpr (LDS) {
+        const double* g = (const double*)a0.uin;
+        for (int c = threadIdx.x; c < ncell; c += NT)
+#pragma unroll
+            for (int v = 0; v < NV; v++) b0[v * WG1D_LDS_CELLS + c] = g[(size_t)v * a0.plane + c];
+        __syncthreads();
+    }
+    int done = 0;  // uniform: every wavefront takes the same branches
     for (int it = 0; it < cap; it += 2) {
-        if (!euler1d_step<RECON, LIM, FLUX>(a0, &ctl, red, &run, lane, wave)) break;
+        if (!euler1d_step<RECON, LIM, FLUX, LDS>(a0, &ctl, red, &run, lane, wave, b0, b1)) break;
+        done++;
         if (it + 1 >= cap) break;
-        if (!euler1d_step<RECON, LIM, FLUX>(a1, &ctl, red, &run, lane, wave)) break;
+        if (!euler1d_step<RECON, LIM, FLUX, LDS>(a1, &ctl, red, &run, lane, wave, b1, b0)) break;
+        done++;
+    }
+    if constexpr (LDS) {
+        // the state after `done` steps is in b0 (even) or b1 (odd); the host expects it in the
+        // input buffer after an even number of steps and in the other one after an odd number.
+        // Every step ends at a barrier, and one that stops early writes no state after it.
+        const double* src = (done & 1) ? b1 : b0;
+        double* g = (done & 1) ? (double*)a0.uout : (double*)const_cast<void*>(a0.uin);
+        for (int c = threadIdx.x; c < ncell; c += NT)
+#pragma unroll
+            for (int v = 0; v < NV; v++) g[(size_t)v * a0.plane + c] = src[v * WG1D_LDS_CELLS + c];
     }
     if (threadIdx.x == 0) *a0.ctl = ctl;  // the other threads only ever read the LDS copy
 }

@@ -40,18 +40,12 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
     if warmup:
         s.step(warmup)
     # ts_step returns only after the library's stream has drained (it reads the step-control
-    # block back), so the timed region starts and ends with the GPU idle.  torch's synchronize is
-    # added where torch sees the device; its own HIP runtime finds none in a process where the
-    # library's runtime already holds it, and the bracket is then the library's own.
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None
-    sync()
+    # block back), so the timed region starts and ends with the GPU idle: that is the
+    # synchronisation bracket.  torch's GPU runtime is never initialised in this process: torch
+    # ships its own HIP and HSA runtime libraries, and a second runtime opening the device beside
+    # the library's fails to find it, or worse.
     t0 = time.perf_counter()
     info = s.step(steps)
-    sync()
     t1 = time.perf_counter()
     assert info.steps_done == steps, info.steps_done
     elapsed = t1 - t0
@@ -409,8 +403,9 @@ def copy_bandwidth(nbytes=1 << 30, reps=20):
     """Device-to-device copy rate in GB/s (bytes read + bytes written per second) of two 1 GiB
     buffers, four times the 256 MB Infinity Cache, so the copy runs at the rate HBM sustains; it
     is printed next to the vendor peak.  torch times the copies with events on its own stream, in
-    a child process: torch brings its own HIP runtime, which finds no device in a process where
-    the library's runtime already holds it.  None if the child fails."""
+    a child process started before this process makes any HIP call: torch brings its own HIP and
+    HSA runtime libraries, which must not open the device beside the library's in one process.
+    None if the child fails."""
     import subprocess
     try:
         r = subprocess.run([sys.executable, "-c", _COPY_PROBE.format(nbytes=nbytes, reps=reps)],
@@ -455,6 +450,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     prec = capi.FP64 if args.precision == "fp64" else capi.FP32_RESIDUAL
+    # first, while this process has made no HIP call: the child is then not a fork of a process
+    # holding the device
+    copy_gbs = copy_bandwidth()
     cfg, elapsed, info, kern, state, (t_set, t_get) = gpu_run(args.n, args.steps, args.warmup, prec)
     ncell = args.n * args.n
     value = ncell * args.steps / elapsed
@@ -462,7 +460,6 @@ def main():
     dname, d = dom
     traffic = traffic_from_profiles(dname)
     bytes_step = sum(s.get("algorithmic_bytes", 0) for s in kern.values()) / ncell
-    copy_gbs = copy_bandwidth()
     out = {
         "metric": "cell-updates/s & % HBM roofline; served-solve p50/p99 at L1 err<=1e-5",
         "value": value,

@@ -214,6 +214,47 @@ def test_server_thread_equals_driven_batch():
         np.testing.assert_array_equal(got[t2][0], ref[t1][0])
 
 
+def test_new_families_arrive_while_the_server_runs():
+    """Solves of configuration families the batch has not seen yet (1-D Godunov and Roe, FP64 and
+    FP32 residual, 2-D HLLC) are submitted one by one while the server thread advances the ones
+    already admitted: every solve finishes with the same bits as in a batch driven by
+    ts_batch_run."""
+    import time
+    probs = []
+    for i in range(3):
+        probs.append(tube(200 + 37 * i, 10 + i))
+        probs.append(tube(150 + 11 * i, 20 + i, flux=capi.RIEMANN_ROE, precision=capi.FP32_RESIDUAL))
+        probs.append(quad(48 + 8 * i, 40 + 5 * i, 30 + i, precision=capi.FP32_RESIDUAL))
+        probs.append(quad(40 + 4 * i, 56, 40 + i))
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    tk = [b.submit(cfg, U, 40) for cfg, U in probs]
+    ref = drain(b, list(tk))
+    b.close()
+    s = capi.Batch(hbm_budget_bytes=1 << 28)
+    s.start_server()
+    try:
+        tk2 = []
+        for cfg, U in probs:
+            tk2.append(s.submit(cfg, U, 40))
+            time.sleep(0.003)
+        got = {}
+        deadline = time.time() + 60
+        while len(got) < len(tk2) and time.time() < deadline:
+            for t in tk2:
+                if t not in got:
+                    d, st, info = s.poll(t)
+                    if d:
+                        got[t] = (st, info)
+            time.sleep(0.001)
+        assert len(got) == len(tk2), "server did not finish the solves"
+    finally:
+        s.stop_server()
+        s.close()
+    for t1, t2 in zip(tk, tk2):
+        assert got[t2][1].steps_done == ref[t1][1].steps_done == 40
+        np.testing.assert_array_equal(got[t2][0], ref[t1][0])
+
+
 @pytest.mark.parametrize("kw", [dict(viscous=True, viscosity=0.01), dict(sigma=(1.0, 0.0, 0.0)),
                                 dict(accel=(0.0, -1.0, 0.0)), dict(viscosity=0.01)])
 def test_batch_refuses_viscous_and_force_terms(kw):

@@ -144,6 +144,8 @@ struct ts_batch {
     long long next_ticket = 1;
     std::deque<long long> queue;
     std::unordered_map<long long, std::unique_ptr<Problem>> probs;
+    // capacity kMaxFamilies is reserved at creation and never exceeded: a submit that adds a
+    // family (under the lock) then never moves the ones the server thread works on without it
     std::vector<Family> families;
     std::vector<double> stage;  // host staging for state conversion
     StepCtl* h_ctl = nullptr;   // pinned, one per resident problem
@@ -207,6 +209,9 @@ Layout layout_of(const ts_config& c) {
     return L;
 }
 
+// every (ndims, flux, reconstruction variant, precision) key: 3 x 3 x 5 x 2
+constexpr size_t kMaxFamilies = 90;
+
 int family_of(ts_batch* b, const ts_config& c) {
     FamilyKey k(c.ndims, c.riemann, tsr::recon_variant(c), c.precision);
     for (size_t i = 0; i < b->families.size(); i++)
@@ -215,6 +220,7 @@ int family_of(ts_batch* b, const ts_config& c) {
     f.key = k;
     if (!tsr::lookup_kernels(c.ndims, c.riemann, tsr::recon_variant(c), c.precision, &f.ks) || !f.ks.bf1) return -1;
     f.occupancy = f.ks.bocc ? f.ks.bocc() : 1;
+    if (b->families.size() >= kMaxFamilies) return -1;
     b->families.push_back(f);
     return (int)b->families.size() - 1;
 }
@@ -503,6 +509,7 @@ int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out) {
     if (!out) return TS_ERR_INVALID;
     *out = nullptr;
     ts_batch* b = new ts_batch();
+    b->families.reserve(kMaxFamilies);
     if (bcfg) b->bc = *bcfg;
     if (b->bc.hbm_budget_bytes == 0) b->bc.hbm_budget_bytes = (size_t)8 << 30;
     if (b->bc.max_active <= 0) b->bc.max_active = 1 << 20;
@@ -603,6 +610,7 @@ int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
     if (!b || n_steps < 0) return TS_ERR_INVALID;
     BCHK(hipSetDevice(b->bc.device));
     std::vector<Problem*> resident;
+    size_t nfam = 0;  // families that exist now; later ones are appended without moving these
     {
         std::lock_guard<std::mutex> lk(b->mu);
         // admit queued solves in arrival order while they fit the budget
@@ -628,9 +636,11 @@ int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
         }
         for (Family& f : b->families)
             for (long long t : f.members) resident.push_back(b->probs[t].get());
+        nfam = b->families.size();
     }
     if (!resident.empty() && n_steps > 0) {
-        for (Family& f : b->families) {
+        for (size_t fi = 0; fi < nfam; fi++) {
+            Family& f = b->families[fi];
             if (f.members.empty()) continue;
             if (f.dirty) {
                 int r = rebuild_tables(b, f);

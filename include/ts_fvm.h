@@ -197,6 +197,8 @@ typedef struct ts_batch_config {
 
 int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out);
 int ts_batch_destroy(ts_batch* b);
+/* The text of the batch's last error, copied for the calling thread: valid until that thread
+   calls this function again. */
 const char* ts_batch_last_error(const ts_batch* b);
 
 /* Queue one solve.  init_state is the reference AoS array of cfg's grid.  The solve runs until

@@ -135,6 +135,7 @@ struct Family {
 struct ts_batch {
     ts_batch_config bc{};
     std::string err;
+    std::mutex err_mu;  // err is written by the caller's thread and by the server thread
     std::mutex mu;
     std::condition_variable cv;
     hipStream_t stream = nullptr;
@@ -162,6 +163,7 @@ struct ts_batch {
 namespace {
 
 int fail(ts_batch* b, int code, const std::string& m) {
+    std::lock_guard<std::mutex> lk(b->err_mu);
     b->err = m;
     return code;
 }
@@ -170,8 +172,7 @@ int fail(ts_batch* b, int code, const std::string& m) {
     do {                                                \
         hipError_t e_ = (call);                         \
         if (e_ != hipSuccess) {                         \
-            b->err = tsr::hip_msg(e_, #call);           \
-            return TS_ERR_HIP;                          \
+            return fail(b, TS_ERR_HIP, tsr::hip_msg(e_, #call)); \
         }                                               \
     } while (0)
 
@@ -550,7 +551,14 @@ int ts_batch_destroy(ts_batch* b) {
     return TS_OK;
 }
 
-const char* ts_batch_last_error(const ts_batch* b) { return b ? b->err.c_str() : "null batch"; }
+const char* ts_batch_last_error(const ts_batch* b) {
+    if (!b) return "null batch";
+    // a copy per calling thread: the server thread may record a new error meanwhile
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(const_cast<ts_batch*>(b)->err_mu);
+    copy = b->err;
+    return copy.c_str();
+}
 
 int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state, int max_steps,
                     double t_end, long long* ticket) {

@@ -209,7 +209,8 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
 
 /* Admit queued solves that fit the budget, then advance every resident solve by up to n_steps
    steps (continuous batching: solves join and leave between steps). Returns the number of
-   solves still resident or queued in *pending. */
+   solves still resident or queued in *pending.  While the server thread runs
+   (ts_batch_start_server) it is the only caller: this returns TS_ERR_INVALID then. */
 int ts_batch_run(ts_batch* b, int n_steps, int* pending);
 
 /* done = 1 when the ticket finished; then out_state (may be NULL) receives the final state in

@@ -214,6 +214,24 @@ def test_server_thread_equals_driven_batch():
         np.testing.assert_array_equal(got[t2][0], ref[t1][0])
 
 
+def test_run_is_refused_while_the_server_runs():
+    """ts_batch_run returns TS_ERR_INVALID while the server thread advances the batch, and works
+    again after ts_batch_stop_server."""
+    cfg, U = tube(120, 5)
+    b = capi.Batch(hbm_budget_bytes=1 << 28)
+    b.start_server()
+    try:
+        with pytest.raises(capi.TSError) as e:
+            b.run(1)
+        assert e.value.code == capi.TS_ERR_INVALID
+    finally:
+        b.stop_server()
+    t = b.submit(cfg, U, 10)
+    out = drain(b, [t])
+    assert out[t][1].steps_done == 10
+    b.close()
+
+
 def test_new_families_arrive_while_the_server_runs():
     """Solves of configuration families the batch has not seen yet (1-D Godunov and Roe, FP64 and
     FP32 residual, 2-D HLLC) are submitted one by one while the server thread advances the ones

@@ -614,8 +614,9 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
     return TS_OK;
 }
 
-int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
-    if (!b || n_steps < 0) return TS_ERR_INVALID;
+// one admission + advance pass; the caller is either the server thread or, when no server
+// runs, the user through ts_batch_run
+static int batch_run(ts_batch* b, int n_steps, int* pending) {
     BCHK(hipSetDevice(b->bc.device));
     std::vector<Problem*> resident;
     size_t nfam = 0;  // families that exist now; later ones are appended without moving these
@@ -709,6 +710,12 @@ int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
     return TS_OK;
 }
 
+int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
+    if (!b || n_steps < 0) return TS_ERR_INVALID;
+    if (b->serving) return fail(b, TS_ERR_INVALID, "the server thread advances this batch: submit and poll instead");
+    return batch_run(b, n_steps, pending);
+}
+
 int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state, ts_step_info* info) {
     if (!b || !done) return TS_ERR_INVALID;
     std::lock_guard<std::mutex> lk(b->mu);
@@ -743,7 +750,7 @@ int ts_batch_start_server(ts_batch* b) {
                 }
             }
             int pending = 0;
-            if (ts_batch_run(b, b->steps_per_chunk, &pending) != TS_OK) {
+            if (batch_run(b, b->steps_per_chunk, &pending) != TS_OK) {
                 std::this_thread::sleep_for(std::chrono::milliseconds(1));
             }
         }

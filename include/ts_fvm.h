@@ -132,6 +132,8 @@ typedef struct ts_ctx ts_ctx;
    device state.  Replaces Kernel::Init + InitializeMethod (kernel.h:178-281, ERK:39-75). */
 int ts_create(const ts_config* cfg, ts_ctx** out);
 int ts_destroy(ts_ctx* ctx);
+/* Message of the context's last failure.  ts_last_error(NULL) returns the message of the calling
+   thread's last failed ts_create (no context exists then to hold it). */
 const char* ts_last_error(const ts_ctx* ctx);
 
 /* Number of doubles of the reference state array (nXAll*nYAll*nZAll*5, kernel.h:240). */

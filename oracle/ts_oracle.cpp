@@ -9,11 +9,13 @@
  *
  * Parity status per function (how each is pinned, see tests/test_oracle_*.py):
  *   roe_flux           consistency, rotation, upwinding, Rankine-Hugoniot, dt0 closed form,
- *                      Sod conservation invariants, long-double brute force
+ *                      Sod conservation invariants; Harten's entropy fix (harten) by the closed
+ *                      form of a resting and a slowly moving contact at Eps = 0.05 and 0.2
  *   exact star state   Toro's star table held by the reference (ToroTests.h:148-191),
  *                      f(p*) = 0 residual, vacuum closed form
- *   hllc_flux          consistency, contact preservation, supersonic upwinding, Toro tests
- *                      1-5 positivity and first-order convergence (extension: no reference)
+ *   hllc_flux          consistency, contact preservation, supersonic upwinding, rotation,
+ *                      Toro tests 1-5: positive density and pressure, L1 density error falling
+ *                      at a first-order rate (extension: no reference)
  *   reconstructions    linear exactness, ENO smallest-slope rule, limiter bounds, BJ / Venkat
  *                      closed forms on 3-cell stencils
  *   boundary images    per-type closed forms (BCGeneral.h:186-268)
@@ -501,11 +503,11 @@ void reconstruct(int recon, int limiter, int nd, const double* v, const double (
         const double* p = nb[2 * a + 1];
         double xm = nbc[2 * a][a], xp = nbc[2 * a + 1][a], x0 = xc[a];
         for (int i = 0; i < NV; i++) {
-            if (recon == TS_RECON_ENO2) { /* ENO2PointsStencil.h:207-235 */
+            if (recon == TS_RECON_ENO2) { /* ENO2PointsStencil.h:21-58 */
                 double gl = (v[i] - m[i]) / (x0 - xm);
                 double gr = (p[i] - v[i]) / (xp - x0);
                 r.g[i][a] = (std::fabs(gl) < std::fabs(gr)) ? gl : gr;
-            } else { /* Linear2PointsStencil.h:147-178 */
+            } else { /* Linear2PointsStencil.h:66-97 */
                 r.g[i][a] = (p[i] - m[i]) / (xp - xm);
             }
         }
@@ -524,7 +526,7 @@ void reconstruct(int recon, int limiter, int nd, const double* v, const double (
             }
         }
     }
-    /* Limiter::ComputeLimiterValues (Limiters.h:162-201) */
+    /* Limiter::ComputeLimiterValues (Limiters.h:28-67) */
     for (int i = 0; i < NV; i++) {
         double umax = v[i], umin = v[i];
         for (int n = 0; n < 2 * nd; n++) {
@@ -539,9 +541,9 @@ void reconstruct(int recon, int limiter, int nd, const double* v, const double (
             double d2 = proj[j][i];
             if (d2 > 0 || d2 < 0) {
                 double um = (d2 > 0) ? umax : umin;
-                if (limiter == TS_LIMITER_BJ) { /* Limiters.h:214-216 */
+                if (limiter == TS_LIMITER_BJ) { /* Limiters.h:80-82 */
                     lv = std::min(1.0, (um - v[i]) / d2);
-                } else { /* Limiters.h:236-249 */
+                } else { /* Limiters.h:102-115 */
                     double un = 0.5 * (umax - umin);
                     double umn = um / un, uin = v[i] / un, dn = d2 / un;
                     double dm = umn - uin;
@@ -723,7 +725,7 @@ struct tso_ctx {
                 double hc[3] = {H[0][i], H[1][j], H[2][k]};
                 double eps2 = 0;
                 if (cfg.recon == TS_RECON_LINEAR_LIM && cfg.limiter == TS_LIMITER_VENKAT) {
-                    /* limVenkatar::SendCellInfo (Limiters.h:227-230) */
+                    /* limVenkatar::SendCellInfo (Limiters.h:93-96) */
                     double dx = std::pow(hc[0] * hc[1] * hc[2], 1.0 / nd);
                     eps2 = std::pow(K * dx, 3.0);
                 }

@@ -67,6 +67,34 @@ def test_invalid_configs_rejected_before_device_use(bad):
     assert not h.value
 
 
+@pytest.mark.parametrize("bad,text", [
+    (dict(precision=5), "unknown precision"), (dict(riemann=7), "unknown Riemann solver"),
+    (dict(nx=1), "at least 2 cells"), (dict(gamma=1.0), "gamma must exceed 1"),
+])
+def test_failed_create_is_readable_through_a_null_context(bad, text):
+    """ts_create returns no context on failure; ts_last_error(NULL) gives this thread's message
+    (and the Python binding raises with it)"""
+    lib = capi.load()
+    kw = dict(ndims=1, nx=10)
+    kw.update(bad)
+    cfg = capi.make_config(**kw)
+    h = C.c_void_p()
+    assert lib.ts_create(C.byref(cfg), C.byref(h)) != capi.TS_OK
+    msg = lib.ts_last_error(None).decode()
+    assert msg.startswith("ts_create:") and text in msg, msg
+    with pytest.raises(capi.TSError, match=text):
+        capi.Solver(cfg)
+
+
+def test_cell_count_limit_of_the_plane_offsets():
+    """a state plane is addressed with 32-bit byte offsets: 2^29 cells and more are refused"""
+    lib = capi.load()
+    cfg = capi.make_config(ndims=3, nx=1024, ny=1024, nz=512)
+    h = C.c_void_p()
+    assert lib.ts_create(C.byref(cfg), C.byref(h)) == capi.TS_ERR_UNSUPPORTED
+    assert "2^29" in lib.ts_last_error(None).decode()
+
+
 @pytest.mark.parametrize("kw,code", [
     (dict(ndims=3, nx=4, ny=4, nz=4, precision=capi.FP32_RESIDUAL, viscous=True, viscosity=0.1),
      capi.TS_ERR_UNSUPPORTED),

@@ -1,5 +1,7 @@
 """Batched solves (persistent kernels over a (problem, tile) table): parity with single-solve
 runs and with the oracle, and the same result alone and in a mix."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -285,4 +287,60 @@ def test_batch_refuses_viscous_and_force_terms(kw):
     with pytest.raises(capi.TSError) as e:
         b.submit(cfg2, U, 10)
     assert e.value.code == capi.TS_ERR_UNSUPPORTED
+    b.close()
+
+
+@pytest.mark.parametrize("kw,code,text", [
+    (dict(precision=7), capi.TS_ERR_INVALID, "unknown precision"),
+    (dict(bcs=[dict(type=9)]), capi.TS_ERR_INVALID, "unknown BC type"),
+    (dict(riemann=5), capi.TS_ERR_INVALID, "unknown Riemann solver"),
+    (dict(rk_stages=2), capi.TS_ERR_INVALID, "rk_stages"),
+])
+def test_batch_submit_applies_the_create_checks(kw, code, text):
+    """ts_batch_submit validates like ts_create: an unknown precision, Riemann solver or boundary
+    type is refused at submission (not run on the wrong kernels or as a Natural boundary)"""
+    base = dict(ndims=2, nx=16, ny=12, cfl=0.45, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                rk_stages=3)
+    base.update(kw)
+    cfg = capi.make_config(**base)
+    U = np.ones(int(np.prod(capi.state_shape(cfg))))
+    b = capi.Batch(hbm_budget_bytes=1 << 26)
+    with pytest.raises(capi.TSError, match=text) as e:
+        b.submit(cfg, U, 10)
+    assert e.value.code == code
+    b.close()
+
+
+def test_batch_submit_refuses_the_plane_offset_limit():
+    """2^29 cells and more would overflow the 32-bit plane offsets: refused at submission (the
+    state pointer is not read before the check)"""
+    cfg = capi.make_config(ndims=2, nx=1 << 15, ny=1 << 14, rk_stages=3, riemann=capi.RIEMANN_HLLC)
+    b = capi.Batch(hbm_budget_bytes=1 << 26)
+    t = C.c_longlong()
+    rc = b.lib.ts_batch_submit(b.h, C.byref(cfg), (C.c_double * 5)(), 10, 0.0, C.byref(t))
+    assert rc == capi.TS_ERR_UNSUPPORTED
+    assert "2^29" in b.lib.ts_batch_last_error(b.h).decode()
+    b.close()
+
+
+def test_batched_venkat_stretched_equals_single_solve():
+    """Venkatakrishnan on a stretched grid in the batch: the per-cell eps^2 factors travel with the
+    problem's geometry, and the batched solve equals the same solve alone (bitwise)"""
+    x, h = util.stretched(240, 1.0, 1.03)
+    y, hy = util.stretched(60, 1.0, 1.05)
+
+    def make():
+        cfg = capi.make_config(ndims=2, nx=240, ny=60, cfl=0.45, eps=0.05, riemann=capi.RIEMANN_ROE,
+                               recon=capi.RECON_LINEAR_LIM, limiter=capi.LIMITER_VENKAT, rk_stages=3,
+                               coord=[x.copy(), y.copy(), None], h=[h.copy(), hy.copy(), None])
+        return cfg, util.quadrants(cfg)
+
+    cfg, U = make()
+    ref, rinfo = run_alone(cfg, U, 30)
+    b = capi.Batch(hbm_budget_bytes=1 << 26)
+    cfg2, U2 = make()
+    t = b.submit(cfg2, U2, 30)
+    st, info = drain(b, [t])[t]
+    assert info.steps_done == 30 and info.time == rinfo.time
+    np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, ref))
     b.close()

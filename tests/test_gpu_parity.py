@@ -76,6 +76,56 @@ def test_sod_reference_config():
     assert abs(dg[0] - 0.4 * 1e-3 / (2 * np.sqrt(1.4))) <= 1e-15
 
 
+@pytest.mark.parametrize("nd", [1, 2, 3])
+@pytest.mark.parametrize("stages", [1, 3])
+def test_venkat_on_stretched_grids(nd, stages):
+    """Limited linear reconstruction with the Venkatakrishnan limiter on geometrically stretched
+    axes: eps^2 = (K (hx hy hz)^(1/nd))^3 of each cell (Limiters.h:93-96), Roe with Eps = 0.05"""
+    n = {1: (120, 1, 1), 2: (70, 44, 1), 3: (37, 24, 18)}[nd]
+    q = (1.04, 1.06, 1.05)
+    geo = [util.stretched(n[a], 1.0, q[a]) if a < nd else (None, None) for a in range(3)]
+    cfg = capi.make_config(ndims=nd, nx=n[0], ny=n[1], nz=n[2], cfl=0.4, eps=0.05, riemann=capi.RIEMANN_ROE,
+                           recon=capi.RECON_LINEAR_LIM, limiter=capi.LIMITER_VENKAT, rk_stages=stages,
+                           coord=[g[0] for g in geo], h=[g[1] for g in geo])
+    U = util.sod(cfg, axis=nd - 1)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 25)
+    check(a, b, dg, dr, gi, ri)
+
+
+def test_reference_sod_scenario_runs_as_configured():
+    """RunSODTestRoe1D (tests.h:13-80) exactly: 100 cells on the two-block compressed grid,
+    Linear2psLim<limVenkatar>, Roe with Eps 0.05, forward Euler, CFL 0.5, Natural BCs, MaxTime 0.25
+    with SaveSolutionTime 0.25 (the last dt lands on 0.25)."""
+    X, H = reference_sod_axis()
+    cfg = capi.make_config(ndims=1, nx=100, cfl=0.5, eps=0.05, riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM,
+                           limiter=capi.LIMITER_VENKAT, coord=[X, None, None], h=[H, None, None])
+    U = util.sod(cfg)
+    a, b, dg, dr, gi, ri = run_pair(cfg, U, 100000, t_max=0.25, snap=0.25)
+    assert gi.time == 0.25 and ri.time == 0.25
+    check(a, b, dg, dr, gi, ri)
+
+
+def reference_sod_axis():
+    """the block compression of RunSODTestRoe1D (tests.h:21-31) built as grid.h:164-218 does:
+    50 cells shrinking by 1/1.05 up to x = 0.5, then 50 growing by 1.05"""
+    def block_step(length, n, q):
+        return length / n if q == 1.0 else length * (1.0 - q) / (1.0 - q ** n)
+    X, H = [], []
+    stp = block_step(0.5, 50, 1 / 1.05)
+    xc = -stp + 0.5 * stp
+    X.append(xc); H.append(stp); xc += stp
+    for q, pos_next, n_next, q_next in ((1 / 1.05, 0.5, 50, 1.05), (1.05, None, 0, None)):
+        for _ in range(50):
+            X.append(xc); H.append(stp)
+            xc += 0.5 * stp * (1.0 + q)
+            stp *= q
+        if pos_next is not None:
+            stp = block_step(1.0 - pos_next, n_next, q_next)
+            xc = pos_next + 0.5 * stp
+    X.append(xc); H.append(stp)
+    return np.array(X), np.array(H)
+
+
 def smooth_1d(g):
     """smooth wave plus a weak density jump: stays physical for every scheme, including the
     unlimited central-linear one with forward Euler"""

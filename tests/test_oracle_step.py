@@ -1,7 +1,7 @@
 """Pins of the oracle's reconstruction, boundary images and whole step (CPU).
 
 Reconstruction: exactness on linear data, the ENO smallest-|slope| rule without a sign check
-(ENO2PointsStencil.h:211), limiter bounds and closed forms.  Whole step: the Sod conservation
+(ENO2PointsStencil.h:30), limiter bounds and closed forms.  Whole step: the Sod conservation
 invariants (mass and energy constant, momentum grows by exactly (pL - pR) dt per step while the
 waves are inside), first-order convergence to the exact solution, x/y/z direction invariance,
 periodic conservation and the diagonal symmetry of Lax-Liu configuration 3."""

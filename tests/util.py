@@ -73,6 +73,31 @@ def quadrants(cfg, states=LAX_LIU_3, xc=0.5, yc=0.5):
     return fill_state(cfg, f)
 
 
+def stretched(n, L, q):
+    """two geometric blocks meeting at L/2 (ratio q, then 1/q), ghost sizes continuing the
+    progression: cell centres and sizes, ghost layer included"""
+    half = n // 2
+    hb = q ** np.arange(half)
+    hb *= 0.5 * L / hb.sum()
+    h = np.concatenate([[hb[0] / q], hb, hb[::-1], [hb[0] / q]])
+    faces = np.concatenate([[0.0], np.cumsum(h[1:-1])])
+    x = np.empty(n + 2)
+    x[1:-1] = 0.5 * (faces[:-1] + faces[1:])
+    x[0] = x[1] - 0.5 * (h[0] + h[1])
+    x[-1] = x[-2] + 0.5 * (h[-2] + h[-1])
+    return x, h
+
+
+def elementwise_rel(a, b, floor):
+    """max over cells of |a-b| / max(|b|, floor[v]) per conserved variable: the error of every
+    cell against its own magnitude, with an absolute floor per variable for values that are zero
+    (or nearly so) by symmetry"""
+    a = np.asarray(a).reshape(-1, 5)
+    b = np.asarray(b).reshape(-1, 5)
+    den = np.maximum(np.abs(b), np.asarray(floor, dtype=np.float64)[None, :])
+    return (np.abs(a - b) / den).max(axis=0)
+
+
 def normwise_rel(a, b):
     """max |a-b| / scale per conserved variable over the interior.  The scale of rho and rho*E
     is their own max |b|; the three momentum components share the scale of the momentum vector

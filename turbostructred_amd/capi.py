@@ -246,7 +246,9 @@ class Solver:
         h = C.c_void_p()
         rc = self.lib.ts_create(C.byref(cfg), C.byref(h))
         if rc != TS_OK:
-            raise TSError(rc, "ts_create failed")
+            self.h = None
+            msg = self.lib.ts_last_error(None)  # this thread's failed ts_create
+            raise TSError(rc, msg.decode() if msg else "ts_create failed")
         self.h = h
 
     def _check(self, rc):

@@ -118,10 +118,10 @@ enum : int { LIM_BJ = 0, LIM_VENKAT = 1 };
 template <int ND, typename R>
 struct CellGeo {
     R hh[ND];   // 0.5 * h (face sample offset, Linear2psLim.h:77-89, ERK:1194-1195)
-    R rc[ND];   // 1 / (x_{i+1} - x_{i-1})  (Linear2PointsStencil.h:153)
-    R rl[ND];   // 1 / (x_i - x_{i-1})      (ENO2PointsStencil.h:209)
-    R rr[ND];   // 1 / (x_{i+1} - x_i)      (ENO2PointsStencil.h:210)
-    R eps2;     // Venkatakrishnan (K dx)^3 (Limiters.h:227-230)
+    R rc[ND];   // 1 / (x_{i+1} - x_{i-1})  (Linear2PointsStencil.h:72)
+    R rl[ND];   // 1 / (x_i - x_{i-1})      (ENO2PointsStencil.h:28)
+    R rr[ND];   // 1 / (x_{i+1} - x_i)      (ENO2PointsStencil.h:29)
+    R eps2;     // Venkatakrishnan (K dx)^3 (Limiters.h:93-96)
 };
 
 // Reconstruct one cell and return its face states: face[2a] on the minus face of axis a,
@@ -134,24 +134,24 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                                    const CellGeo<ND, R>& geo, R (&face)[2 * ND][NV]) {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        if constexpr (RECON == RC_PWC) {  // PiecewiseConstant.h:57
+        if constexpr (RECON == RC_PWC) {  // PiecewiseConstant.h:15
 #pragma unroll
             for (int f = 0; f < 2 * ND; f++) face[f][v] = c[v];
         } else {
             R g[ND];
 #pragma unroll
             for (int a = 0; a < ND; a++) {
-                if constexpr (RECON == RC_ENO2) {  // ENO2PointsStencil.h:207-235
+                if constexpr (RECON == RC_ENO2) {  // ENO2PointsStencil.h:21-58
                     R gl = (c[v] - nb[2 * a][v]) * geo.rl[a];
                     R gr = (nb[2 * a + 1][v] - c[v]) * geo.rr[a];
                     g[a] = (__builtin_fabs(gl) < __builtin_fabs(gr)) ? gl : gr;
-                } else {  // Linear2PointsStencil.h:147-178
+                } else {  // Linear2PointsStencil.h:66-97
                     g[a] = (nb[2 * a + 1][v] - nb[2 * a][v]) * geo.rc[a];
                 }
             }
             R lim = R(1);
             if constexpr (RECON == RC_LINLIM) {
-                // Limiter::ComputeLimiterValues (Limiters.h:162-201).  The minimum over the
+                // Limiter::ComputeLimiterValues (Limiters.h:28-67).  The minimum over the
                 // faces of min(1, l_j) is found by comparing fractions A/D, then divided once.
                 R umax = c[v], umin = c[v];
 #pragma unroll
@@ -161,7 +161,7 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                 }
                 if (!(umax - umin < tiny_spread<R>())) {
                     if constexpr (LIM == LIM_BJ) {
-                        // Barth-Jespersen (Limiters.h:214-216).  Along each axis the two face
+                        // Barth-Jespersen (Limiters.h:80-82).  Along each axis the two face
                         // projections are +-d_a, one bounded by umax and the other by umin, so
                         // min over faces of (u_m - u_i)/d2 = min(umax - c, c - umin) / max_a |d_a|.
                         R m = __builtin_fmin(umax - c[v], c[v] - umin);
@@ -199,7 +199,7 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                                     d2 = fv - c[v];
                                 }
                                 if (d2 > R(0) || d2 < R(0)) {
-                                    // Venkatakrishnan (Limiters.h:236-249), normalised by
+                                    // Venkatakrishnan (Limiters.h:102-115), normalised by
                                     // (umax - umin)/2
                                     R um = (d2 > R(0)) ? umax : umin;
                                     R dm = um * inv_un - c[v] * inv_un;
@@ -221,7 +221,7 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                 if constexpr (RECON == RC_LINLIM) {  // Linear2psLim.h:17-21
                     face[2 * a][v] = c[v] + lim * dl;
                     face[2 * a + 1][v] = c[v] + lim * dr;
-                } else {  // Linear2PointsStencil.h:107-111
+                } else {  // Linear2PointsStencil.h:26-30
                     face[2 * a][v] = c[v] + dl;
                     face[2 * a + 1][v] = c[v] + dr;
                 }

@@ -66,7 +66,10 @@ struct GridArgs {
     const double* rl[3];  // 1/(x_i - x_{i-1})
     const double* rr[3];  // 1/(x_{i+1} - x_i)
     const double* ih[3];  // 1/h, ghost layer included
-    double eps2;          // Venkatakrishnan eps^2 (uniform grids)
+    double eps2;          // Venkatakrishnan eps^2 of every cell (uniform grids)
+    // non-uniform grids: eps^2 of cell (i, j, k) = ev[0][i] * ev[1][j] * ev[2][k], the per-axis
+    // factors (K h_a)^(3/nd) of (K (hx hy hz)^(1/nd))^3 (Limiters.h:93-96); null when uniform
+    const double* ev[3];
     double gamma, eps, cfl;
     BCDev bc[MAX_BC];
     int side_marker[6];
@@ -367,6 +370,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     geo.rl[0] = R(g.rl[0][xs]);
     geo.rr[0] = R(g.rr[0][xs]);
     geo.eps2 = R(g.eps2);
+    constexpr bool VENKAT = RECON == RC_LINLIM && LIM == LIM_VENKAT;
+    // per-cell eps^2 on non-uniform grids (a wave-uniform branch, Venkatakrishnan instances only)
+    const bool ev_cells = VENKAT && g.ev[0] != nullptr;
+    const double ev_x = ev_cells ? g.ev[0][xs] : 0.0;
+    if (ev_cells && DIM == 1) geo.eps2 = R(ev_x);
 
     const TIn* __restrict__ uin = LDS_STATE ? (const TIn*)uin_arg : (const TIn*)a.uin;
     const TOut* __restrict__ lin = (const TOut*)a.lin;
@@ -538,6 +546,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
             geo.rc[1] = R(g.rc[1][rcs]);
             geo.rl[1] = R(g.rl[1][rcs]);
             geo.rr[1] = R(g.rr[1][rcs]);
+            if (ev_cells) geo.eps2 = R(ev_x * g.ev[1][rcs]);
             R nb[4][NV], face[4][NV];
             {
                 R U0[NV];

@@ -76,6 +76,9 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     geo.rl[2] = R(g.rl[2][ks]);
     geo.rr[2] = R(g.rr[2][ks]);
     geo.eps2 = R(g.eps2);
+    // per-cell Venkatakrishnan eps^2 on non-uniform grids (see GridArgs::ev)
+    const bool ev_cells = RECON == RC_LINLIM && LIM == LIM_VENKAT && g.ev[0] != nullptr;
+    const double ev_xz = ev_cells ? g.ev[0][xs] * g.ev[2][ks] : 0.0;
 
     const TIn* __restrict__ uin = (const TIn*)a.uin;
     const TOut* __restrict__ lin = (const TOut*)a.lin;
@@ -161,6 +164,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         geo.rc[1] = R(g.rc[1][rcs]);
         geo.rl[1] = R(g.rl[1][rcs]);
         geo.rr[1] = R(g.rr[1][rcs]);
+        if (ev_cells) geo.eps2 = R(ev_xz * g.ev[1][rcs]);
 
         // (1) share the centre row, reconstruct
 #pragma unroll

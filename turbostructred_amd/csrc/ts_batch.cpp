@@ -111,7 +111,7 @@ struct Problem {
     int n[3] = {1, 1, 1};
     std::vector<double> result;
     ts_step_info info{};
-    int status = 0;  // 0 queued, 1 resident, 2 done
+    int status = 0;  // 0 queued, 1 resident (or being admitted), 2 done
     int error = 0;
 };
 
@@ -124,7 +124,10 @@ struct Family {
     int2* d_table = nullptr;
     int* d_ntiles = nullptr;  // tiles in the table (read by the kernels)
     size_t cap_desc = 0, cap_table = 0;
-    std::vector<long long> members;  // tickets of resident problems, in descriptor order
+    // resident problems in descriptor order.  A Problem is owned by ts_batch::probs through a
+    // unique_ptr, so its address is stable: the run thread reads members without the map (and
+    // without the lock), while submit inserts into and poll erases from the map under the lock.
+    std::vector<Problem*> members;
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -132,11 +135,20 @@ struct Family {
 
 }  // namespace
 
+// Locking.  `mu` guards the queue, the ticket map, the arena and the family list's length; it
+// is held only for bookkeeping, never across a device copy or synchronisation.  `run_mu` is held
+// for the whole of one admission + advance pass (batch_run), by the server thread and by
+// ts_batch_run alike, so that two passes never share the stream, the control-block buffers or a
+// family's tables and graph.  The run thread reads Family::members and the Problems in it
+// without `mu`: only a pass (under run_mu) adds or removes members, and poll only erases
+// problems that have left every family.
 struct ts_batch {
     ts_batch_config bc{};
     std::string err;
     std::mutex err_mu;  // err is written by the caller's thread and by the server thread
     std::mutex mu;
+    std::mutex run_mu;
+    std::mutex server_mu;  // serialises ts_batch_start_server / ts_batch_stop_server
     std::condition_variable cv;
     hipStream_t stream = nullptr;
     Arena arena;
@@ -192,7 +204,8 @@ Layout layout_of(const ts_config& c) {
     int na[3] = {c.nx, c.ny, c.nz};
     for (int a = 0; a < 3; a++) {
         int m = a < c.ndims ? na[a] : 1;
-        geo += 2 * (m + 2) + 3 * m;  // h, ih (with ghosts); rc, rl, rr (interior)
+        // h, ih (with ghosts); rc, rl, rr and the Venkatakrishnan factors (interior)
+        geo += 2 * (m + 2) + 4 * m;
     }
     L.geo_doubles = geo;
     size_t o = 0;
@@ -243,8 +256,10 @@ int admit(ts_batch* b, Problem& p) {
     geo.reserve(L.geo_doubles);
     double* dgeo = (double*)(base + L.geo);
     bool uniform = true;
-    std::vector<double> X, H;
+    std::vector<double> X, H, EV;
     double h1[3] = {1.0, 1.0, 1.0};
+    size_t oev[3] = {0, 0, 0};
+    const bool venkat = c.recon == TS_RECON_LINEAR_LIM && c.limiter == TS_LIMITER_VENKAT;
     for (int a = 0; a < 3; a++) {
         int n = a < c.ndims ? p.n[a] : 1;
         if (a < c.ndims) {
@@ -265,18 +280,19 @@ int admit(ts_batch* b, Problem& p) {
         for (int i = 0; i < n; i++) geo.push_back(a < c.ndims ? 1.0 / (X[i + 1] - X[i]) : 0.0);
         size_t orr = geo.size();
         for (int i = 0; i < n; i++) geo.push_back(a < c.ndims ? 1.0 / (X[i + 2] - X[i + 1]) : 0.0);
+        oev[a] = geo.size();
+        tsr::venkat_axis_factors(c.ndims, n, a < c.ndims ? H.data() + 1 : nullptr, EV);
+        for (int i = 0; i < n; i++) geo.push_back(EV.size() == 1 ? EV[0] : EV[i]);
         g.h[a] = dgeo + oh;
         g.ih[a] = dgeo + oi;
         g.rc[a] = dgeo + orc;
         g.rl[a] = dgeo + orl;
         g.rr[a] = dgeo + orr;
     }
-    g.eps2 = 0.0;
-    if (c.recon == TS_RECON_LINEAR_LIM && c.limiter == TS_LIMITER_VENKAT) {
-        if (!uniform) return fail(b, TS_ERR_UNSUPPORTED, "Venkatakrishnan limiter needs a uniform grid");
-        // limVenkatar::SendCellInfo (Limiters.h:227-230), K = 5
-        g.eps2 = std::pow(5.0 * std::pow(h1[0] * h1[1] * h1[2], 1.0 / c.ndims), 3.0);
-    }
+    // limVenkatar::SendCellInfo (Limiters.h:93-96), K = 5: one value on a uniform grid, per-axis
+    // factors of the per-cell value otherwise
+    g.eps2 = venkat ? tsr::venkat_eps2(c.ndims, h1[0], h1[1], h1[2]) : 0.0;
+    for (int a = 0; a < 3; a++) g.ev[a] = (venkat && !uniform) ? dgeo + oev[a] : nullptr;
     for (int m = 0; m < TS_MAX_BC; m++) {
         g.bc[m].type = m < c.n_bc ? c.bc[m].type : TS_BC_NATURAL;
         for (int a = 0; a < 3; a++) g.bc[m].vel[a] = c.bc[m].velocity[a];
@@ -352,14 +368,14 @@ void retile(ts_batch* b, Family& f) {
         int G = 1;
         for (int cand = 8; cand > 1; cand /= 2) {
             long long items = 0;
-            for (long long t : f.members) items += (b->probs[t]->desc.nstrips + cand - 1) / cand;
+            for (const Problem* q : f.members) items += (q->desc.nstrips + cand - 1) / cand;
             if (items >= waves) {
                 G = cand;
                 break;
             }
         }
-        for (long long t : f.members) {
-            Problem& p = *b->probs[t];
+        for (Problem* q : f.members) {
+            Problem& p = *q;
             p.desc.ty = G;
             p.desc.ntiles = p.desc.nstrips;
             p.ntiles = (p.desc.nstrips + G - 1) / G;
@@ -369,8 +385,8 @@ void retile(ts_batch* b, Family& f) {
     int ty = 4;
     for (int cand = 32; cand > 4; cand /= 2) {
         long long tiles = 0;
-        for (long long t : f.members) {
-            const Problem& p = *b->probs[t];
+        for (const Problem* q : f.members) {
+            const Problem& p = *q;
             tiles += (long long)p.desc.nstrips * ((p.n[1] + cand - 1) / cand);
         }
         if (tiles >= waves) {
@@ -378,8 +394,8 @@ void retile(ts_batch* b, Family& f) {
             break;
         }
     }
-    for (long long t : f.members) {
-        Problem& p = *b->probs[t];
+    for (Problem* q : f.members) {
+        Problem& p = *q;
         p.desc.ty = ty;
         p.desc.ntiles = p.desc.nstrips * ((p.n[1] + ty - 1) / ty);
         p.ntiles = p.desc.ntiles;
@@ -390,7 +406,7 @@ int rebuild_tables(ts_batch* b, Family& f) {
     retile(b, f);
     size_t np = f.members.size();
     size_t nt = 0;
-    for (long long t : f.members) nt += b->probs[t]->ntiles;
+    for (const Problem* q : f.members) nt += q->ntiles;
     bool realloc = false;
     if (np > f.cap_desc) {  // cap_desc descriptors per kernel, three kernels
         if (f.d_desc) BCHK(hipFree(f.d_desc));
@@ -418,7 +434,7 @@ int rebuild_tables(ts_batch* b, Family& f) {
     std::vector<int2> table;
     table.reserve(nt);
     for (size_t i = 0; i < np; i++) {
-        const Problem& p = *b->probs[f.members[i]];
+        const Problem& p = *f.members[i];
         for (int k = 0; k < 3; k++) desc[k * f.cap_desc + i] = stage_desc(p, k, b->d_arena);
         for (int t = 0; t < p.ntiles; t++) table.push_back(make_int2((int)i, t));
     }
@@ -477,8 +493,9 @@ int start_retire(ts_batch* b, Problem& p) {
     return TS_OK;
 }
 
-// SoA interior -> reference AoS layout (ghosts zero), control fields -> step info; frees the block
-void finish_retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
+// SoA interior -> reference AoS layout (ghosts zero); needs no lock (the problem is still resident,
+// so poll does not read it)
+void convert_retired(Problem& p) {
     const ts_config& c = p.cfg;
     const size_t n = p.ncell;
     int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
@@ -492,6 +509,10 @@ void finish_retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
                 for (int v = 0; v < 5; v++) p.result[dst + v] = p.init[v * n + s];
             }
     std::vector<double>().swap(p.init);
+}
+
+// control fields -> step info, marks the problem done and frees its block (under the lock)
+void finish_retire(ts_batch* b, Problem& p, const StepCtl& ctl) {
     p.info.time = ctl.time;
     p.info.dt = ctl.dt;
     p.info.iteration = ctl.iter;
@@ -563,7 +584,13 @@ const char* ts_batch_last_error(const ts_batch* b) {
 int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state, int max_steps,
                     double t_end, long long* ticket) {
     if (!b || !cfg || !init_state || !ticket || max_steps < 0) return TS_ERR_INVALID;
-    // the same validation as ts_create, then the batch's own limits
+    // the same validation as ts_create (cell-count limit of the 32-bit plane offsets, known
+    // precision, Riemann solver and boundary types, ...), then the batch's own limits
+    {
+        std::string err;
+        int rc = tsr::validate(cfg, err);
+        if (rc != TS_OK) return fail(b, rc, err);
+    }
     if (cfg->ndims < 1 || cfg->ndims > 2) return fail(b, TS_ERR_UNSUPPORTED, "batched solves are 1-D or 2-D");
     if (cfg->rk_stages != 3) return fail(b, TS_ERR_UNSUPPORTED, "batched solves use SSP-RK3");
     if (cfg->viscous || cfg->external_force || cfg->external_accel || cfg->viscosity != 0.0)
@@ -615,38 +642,50 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
 }
 
 // one admission + advance pass; the caller is either the server thread or, when no server
-// runs, the user through ts_batch_run
+// runs, the user through ts_batch_run.  Runs under run_mu.
 static int batch_run(ts_batch* b, int n_steps, int* pending) {
     BCHK(hipSetDevice(b->bc.device));
+    // (1) under the lock: take queued solves off the queue in arrival order while they fit the
+    // budget, reserving their arena blocks (head-of-line waits for memory)
+    std::vector<Problem*> incoming;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        size_t nres = 0;
+        for (Family& f : b->families) nres += f.members.size();
+        while (!b->queue.empty() && (int)(nres + incoming.size()) < b->bc.max_active) {
+            Problem& p = *b->probs[b->queue.front()];
+            size_t off;
+            if (!b->arena.alloc(p.bytes, &off)) break;
+            p.off = off;
+            p.status = 1;
+            b->queue.pop_front();
+            incoming.push_back(&p);
+        }
+    }
+    // (2) without the lock: upload their geometry, state and control blocks, so that submit and
+    // poll on other threads do not wait behind the copies
+    std::vector<int> admitted(incoming.size(), 0);
+    for (size_t i = 0; i < incoming.size(); i++) admitted[i] = admit(b, *incoming[i]) == TS_OK;
+    // (3) under the lock again: attach them to their families (or fail them)
     std::vector<Problem*> resident;
     size_t nfam = 0;  // families that exist now; later ones are appended without moving these
     {
         std::lock_guard<std::mutex> lk(b->mu);
-        // admit queued solves in arrival order while they fit the budget
-        size_t nres = 0;
-        for (Family& f : b->families) nres += f.members.size();
-        while (!b->queue.empty() && (int)nres < b->bc.max_active) {
-            Problem& p = *b->probs[b->queue.front()];
-            size_t off;
-            if (!b->arena.alloc(p.bytes, &off)) break;  // head-of-line waits for memory
-            p.off = off;
-            int r = admit(b, p);
-            if (r) {
-                b->arena.release(off, p.bytes);
+        for (size_t i = 0; i < incoming.size(); i++) {
+            Problem& p = *incoming[i];
+            if (!admitted[i]) {
+                b->arena.release(p.off, p.bytes);
                 p.status = 2;
                 p.error = 1;
-                b->queue.pop_front();
                 continue;
             }
-            b->families[p.family].members.push_back(p.ticket);
+            b->families[p.family].members.push_back(&p);
             b->families[p.family].dirty = true;
-            b->queue.pop_front();
-            nres++;
         }
-        for (Family& f : b->families)
-            for (long long t : f.members) resident.push_back(b->probs[t].get());
         nfam = b->families.size();
     }
+    for (size_t fi = 0; fi < nfam; fi++)
+        for (Problem* q : b->families[fi].members) resident.push_back(q);
     if (!resident.empty() && n_steps > 0) {
         for (size_t fi = 0; fi < nfam; fi++) {
             Family& f = b->families[fi];
@@ -691,13 +730,15 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
             fin.push_back(i);
         }
         if (!fin.empty()) BCHK(hipStreamSynchronize(b->stream));
+        // the layout conversion of the finished states happens before the lock is taken
+        for (int i : fin) convert_retired(*resident[i]);
         std::lock_guard<std::mutex> lk(b->mu);
         for (int i : fin) {
             Problem& p = *resident[i];
-            finish_retire(b, p, b->h_ctl[i]);
             Family& f = b->families[p.family];
-            f.members.erase(std::find(f.members.begin(), f.members.end(), p.ticket));
+            f.members.erase(std::find(f.members.begin(), f.members.end(), &p));
             f.dirty = true;
+            finish_retire(b, p, b->h_ctl[i]);
         }
     }
     if (pending) {
@@ -713,6 +754,8 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
 int ts_batch_run(ts_batch* b, int n_steps, int* pending) {
     if (!b || n_steps < 0) return TS_ERR_INVALID;
     if (b->serving) return fail(b, TS_ERR_INVALID, "the server thread advances this batch: submit and poll instead");
+    // a server started meanwhile waits for this pass: passes never overlap
+    std::lock_guard<std::mutex> run(b->run_mu);
     return batch_run(b, n_steps, pending);
 }
 
@@ -736,8 +779,10 @@ int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state, t
 
 int ts_batch_start_server(ts_batch* b) {
     if (!b) return TS_ERR_INVALID;
-    if (b->serving.exchange(true)) return TS_OK;
+    std::lock_guard<std::mutex> g(b->server_mu);
+    if (b->serving) return TS_OK;
     b->stop = false;
+    b->serving = true;
     b->server = std::thread([b] {
         while (!b->stop) {
             {
@@ -749,10 +794,12 @@ int ts_batch_start_server(ts_batch* b) {
                     continue;
                 }
             }
-            int pending = 0;
-            if (batch_run(b, b->steps_per_chunk, &pending) != TS_OK) {
-                std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            int pending = 0, rc;
+            {
+                std::lock_guard<std::mutex> run(b->run_mu);
+                rc = batch_run(b, b->steps_per_chunk, &pending);
             }
+            if (rc != TS_OK) std::this_thread::sleep_for(std::chrono::milliseconds(1));
         }
     });
     return TS_OK;
@@ -760,10 +807,13 @@ int ts_batch_start_server(ts_batch* b) {
 
 int ts_batch_stop_server(ts_batch* b) {
     if (!b) return TS_ERR_INVALID;
-    if (!b->serving.exchange(false)) return TS_OK;
+    std::lock_guard<std::mutex> g(b->server_mu);
+    if (!b->serving) return TS_OK;
+    // the server finishes its pass and exits before ts_batch_run is allowed again
     b->stop = true;
     b->cv.notify_all();
     if (b->server.joinable()) b->server.join();
+    b->serving = false;
     return TS_OK;
 }
 

@@ -81,6 +81,13 @@ std::string hip_msg(hipError_t e, const char* what);
 void build_axis(const ts_config& c, int a, std::vector<double>& X, std::vector<double>& H);
 bool uniform_axis(const std::vector<double>& H);
 int recon_variant(const ts_config& c);
+// the configuration checks of ts_create: TS_OK, or an error code with its message in err
+int validate(const ts_config* c, std::string& err);
+// Venkatakrishnan eps^2 = (K (hx hy hz)^(1/nd))^3, K = 5 (Limiters.h:93-96, Constants.h:11)
+double venkat_eps2(int nd, double hx, double hy, double hz);
+// per-axis factors (K h_i)^(3/nd) of that value for the n interior cells of an axis whose sizes
+// start at h (null for an inactive axis: one factor 1)
+void venkat_axis_factors(int nd, int n, const double* h, std::vector<double>& out);
 bool lookup_kernels(int dim, int flux, int rv, int precision, tsd::KernelSet* ks);
 void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, double* o);
 // output rows per tile; `slots` = tiles the GPU holds at once (0: unknown)

@@ -106,6 +106,21 @@ int validate(const ts_config* c, std::string& err) {
     return TS_OK;
 }
 
+double venkat_eps2(int nd, double hx, double hy, double hz) {
+    return std::pow(5.0 * std::pow(hx * hy * hz, 1.0 / nd), 3.0);
+}
+
+void venkat_axis_factors(int nd, int n, const double* h, std::vector<double>& out) {
+    if (!h) {
+        out.assign(1, 1.0);
+        return;
+    }
+    out.resize(n);
+    const double e = 3.0 / nd;
+    const double kf = std::pow(5.0, e);
+    for (int i = 0; i < n; i++) out[i] = kf * std::pow(h[i], e);
+}
+
 int recon_variant(const ts_config& c) {
     if (c.recon == TS_RECON_LINEAR_LIM) return c.limiter == TS_LIMITER_VENKAT ? 4 : 3;
     return c.recon;
@@ -175,15 +190,22 @@ static int ctx_fail(ts_ctx* c, int code, const std::string& m) {
     return code;
 }
 
+// Message of the calling thread's last failed ts_create, where no context exists to hold it;
+// ts_last_error(NULL) returns it.
+static thread_local std::string t_create_err;
+
 extern "C" {
 
 int ts_create(const ts_config* cfg, ts_ctx** out) {
-    if (!out) return TS_ERR_INVALID;
+    if (!out) {
+        t_create_err = "ts_create: null output pointer";
+        return TS_ERR_INVALID;
+    }
     *out = nullptr;
     std::string err;
     int rc = validate(cfg, err);
     if (rc) {
-        fprintf(stderr, "ts_create: %s\n", err.c_str());
+        t_create_err = "ts_create: " + err;
         return rc;
     }
     ts_ctx* c = new ts_ctx();
@@ -197,12 +219,13 @@ int ts_create(const ts_config* cfg, ts_ctx** out) {
     c->ncell = (size_t)c->n[0] * c->n[1] * c->n[2];
     int r = init_device_ctx(c, cfg);
     if (r != TS_OK) {
-        fprintf(stderr, "ts_create: %s\n", c->err.c_str());
+        t_create_err = "ts_create: " + c->err;
         destroy_device_ctx(c);
         delete c;
         return r;
     }
     *out = c;
+    t_create_err.clear();
     return TS_OK;
 }
 
@@ -213,7 +236,10 @@ int ts_destroy(ts_ctx* c) {
     return TS_OK;
 }
 
-const char* ts_last_error(const ts_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* ts_last_error(const ts_ctx* c) {
+    if (c) return c->err.c_str();
+    return t_create_err.empty() ? "null context" : t_create_err.c_str();
+}
 
 size_t ts_state_size(const ts_ctx* c) {
     if (!c) return 0;

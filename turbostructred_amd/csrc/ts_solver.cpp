@@ -106,15 +106,19 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         if ((r = upload(c, rr.data(), rr.size(), &g.rr[a]))) return r;
     }
     g.eps2 = 0.0;
+    for (int a = 0; a < 3; a++) g.ev[a] = nullptr;
     if (k.recon == TS_RECON_LINEAR_LIM && k.limiter == TS_LIMITER_VENKAT) {
+        // limVenkatar::SendCellInfo (Limiters.h:93-96) with K = 5 (Constants.h:11): one value on
+        // a uniform grid, per-axis factors of the per-cell value otherwise
+        g.eps2 = tsr::venkat_eps2(c->nd, c->H[0][1], c->nd > 1 ? c->H[1][1] : 1.0, c->nd > 2 ? c->H[2][1] : 1.0);
         if (!uniform) {
-            c->err = "the Venkatakrishnan limiter is built for uniform grids only";
-            return TS_ERR_UNSUPPORTED;
+            for (int a = 0; a < 3; a++) {
+                std::vector<double> f;
+                tsr::venkat_axis_factors(c->nd, a < c->nd ? c->n[a] : 1, a < c->nd ? c->H[a].data() + 1 : nullptr, f);
+                int r;
+                if ((r = upload(c, f.data(), f.size(), &g.ev[a]))) return r;
+            }
         }
-        // limVenkatar::SendCellInfo (Limiters.h:227-230) with K = 5 (Constants.h:11)
-        double hx = c->H[0][1], hy = c->nd > 1 ? c->H[1][1] : 1.0, hz = c->nd > 2 ? c->H[2][1] : 1.0;
-        double dx = std::pow(hx * hy * hz, 1.0 / c->nd);
-        g.eps2 = std::pow(5.0 * dx, 3.0);
     }
     for (int m = 0; m < TS_MAX_BC; m++) {
         const ts_bc_config& b = k.bc[m];

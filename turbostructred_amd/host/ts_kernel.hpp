@@ -55,7 +55,7 @@ struct MethodConfiguration {
     int Precision{TS_FP64};  // TS_FP64 or TS_FP32_RESIDUAL
 };
 
-// src/BoundaryConditions/BoundaryConditionConfiguration.h:12-74
+// src/BoundaryConditions/BoundaryConditionConfiguration.h:12-69
 enum class BoundaryConditionType { Wall, Symmetry, MovingWall, SubsonicInlet, SubsonicOutlet, Natural };
 
 struct BoundaryConditionConfiguration {
@@ -176,6 +176,287 @@ inline void BuildAxis(std::vector<BlockNode> blocks, int n, double L, int ghost,
         X[c + i] = xc;
         H[c + i] = stp;
         xc += stp;
+    }
+}
+
+// ---- equation of state (src/EquationsOfState/generalEOS.h:6-23, idealGasEOS.h:7-46) ------------
+// The host side of the EquationOfState interface.  The device solvers take the same ideal-gas
+// law as a functor (tsd::IdealGasEOS in the library); both are the perfect gas p = (g - 1) rho e
+// of the reference's Roe and Godunov solvers.
+class GeneralEOS {
+   public:
+    virtual ~GeneralEOS() = default;
+    virtual double GetPressure(double ro, double e) = 0;
+    virtual double GetSoundSpeed(double ro, double e) = 0;
+    virtual double GetInternalEnergy(double ro, double pressure) = 0;
+    virtual double GetPressureEnergyDerivative(double ro, double e) = 0;
+    virtual double GetPressureDensityDerivative(double ro, double e) = 0;
+};
+
+class IdealGasEOS : public GeneralEOS {
+   public:
+    double gamma;
+    explicit IdealGasEOS(double g) : gamma(g) {}
+    void SetGamma(double g) { gamma = g; }
+    double GetPressure(double ro, double e) override { return (gamma - 1.0) * ro * e; }
+    double GetSoundSpeed(double ro, double e) override { return std::sqrt(gamma * ((gamma - 1.0) * ro * e) / ro); }
+    double GetInternalEnergy(double ro, double pressure) override { return pressure / ((gamma - 1.0) * ro); }
+    double GetPressureEnergyDerivative(double ro, double) override { return (gamma - 1.0) * ro; }
+    double GetPressureDensityDerivative(double, double e) override { return (gamma - 1.0) * e; }
+};
+
+// ---- cell averages by Gauss quadrature (src/utility/NumericQuadrature.h:18-226) --------------------
+// Gauss-Legendre nodes and weights are computed here (Newton's method on P_n in long double, then
+// rounded), not tabulated; the orders the reference tabulates are accepted (1-12 and 32).  Two
+// behaviours of the reference are kept because they decide the initial state it produces:
+//  * order 10 takes the 12-point half table through the 10-point mirror fill (the missing
+//    `break` of NumericQuadrature.h:183-191): its weights sum to 2 - 2 w_1 of the 12-point rule;
+//  * IntegrateGroap samples point (x_i, y_i, z_i) for the weight wx_i wy_j wz_k (the y and z
+//    coordinates take the x index, NumericQuadrature.h:97-99), so in 2-D and 3-D it integrates
+//    along the cell's diagonal.  Below 3-D the reference reads y/z nodes past their one-element
+//    arrays there; here those coordinates stay at the cell centre.
+class NumericQuadrature {
+   public:
+    NumericQuadrature(int n, int dims) : n_(n), dims_(dims) {
+        const bool ok = (n >= 1 && n <= 12) || n == 32;
+        if (!ok) throw std::invalid_argument("Gauss quadrature order must be 1-12 or 32: " + std::to_string(n));
+        Fill();
+    }
+    int Order() const { return n_; }
+    const std::vector<double>& Nodes() const { return x_; }
+    const std::vector<double>& Weights() const { return wx_; }
+
+    // integral of f (nvars values) over the cell, IntegrateGroap (NumericQuadrature.h:86-118)
+    template <typename F>
+    std::valarray<double> IntegrateGroap(double cx, double cy, double cz, double hx, double hy, double hz, F&& f,
+                                         size_t nvars) const {
+        std::valarray<double> res(0.0, nvars);
+        const int ny = dims_ > 1 ? n_ : 1, nz = dims_ > 2 ? n_ : 1;
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < n_; i++) {
+                    const double rx = cx + x_[i] * 0.5 * hx;
+                    const double ry = cy + (dims_ > 1 ? x_[i] : 0.0) * 0.5 * hy;
+                    const double rz = cz + (dims_ > 2 ? x_[i] : 0.0) * 0.5 * hz;
+                    const std::vector<double> vars = f(Vector(rx, ry, rz));
+                    const double w = wx_[i] * (dims_ > 1 ? wx_[j] : 1.0) * (dims_ > 2 ? wx_[k] : 1.0);
+                    for (size_t v = 0; v < nvars; v++) res[v] += w * vars[v];
+                }
+        double J = 0.5 * hx;  // ComputeJacobian (NumericQuadrature.h:31-37)
+        if (dims_ > 1) J *= 0.5 * hy;
+        if (dims_ > 2) J *= 0.5 * hz;
+        res *= J;
+        return res;
+    }
+
+   private:
+    int n_, dims_;
+    std::vector<double> x_, wx_;
+
+    // nonnegative roots of P_m in ascending order and their weights
+    static void HalfTable(int m, std::vector<double>& r, std::vector<double>& w) {
+        std::vector<std::pair<long double, long double>> rw;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int i = 1; i <= m; i++) {
+            long double x = std::cos(pi * (i - 0.25L) / (m + 0.5L)), dp = 0;
+            for (int it = 0; it < 100; it++) {
+                long double p0 = 1, p1 = x;
+                for (int k = 2; k <= m; k++) {
+                    const long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                if (m == 1) p0 = 1;
+                dp = m * (x * p1 - p0) / (x * x - 1);
+                const long double dx = p1 / dp;
+                x -= dx;
+                if (std::fabs(dx) < 1e-19L) break;
+            }
+            if (x >= -1e-30L) rw.push_back({std::fabs(x) < 1e-30L ? 0.0L : x, 2 / ((1 - x * x) * dp * dp)});
+        }
+        std::sort(rw.begin(), rw.end());
+        r.clear();
+        w.clear();
+        for (auto& q : rw) {
+            r.push_back((double)q.first);
+            w.push_back((double)q.second);
+        }
+    }
+    // FillCoefficients (NumericQuadrature.h:121-226): the mirror fill from the half table
+    void Fill() {
+        std::vector<double> r, w;
+        HalfTable(n_ == 10 ? 12 : n_, r, w);
+        x_.assign(n_, 0.0);
+        wx_.assign(n_, 0.0);
+        const int ws = (int)w.size();
+        for (int i = 0; i < ws; i++) {
+            wx_[i] = w[ws - 1 - i];
+            wx_[n_ - 1 - i] = wx_[i];
+            x_[n_ - 1 - i] = r[ws - 1 - i];
+            x_[i] = -x_[n_ - 1 - i];
+        }
+    }
+};
+
+// ---- Tecplot and slice writers (kernel.h:1081-1632) ---------------------------------------------
+// The grid (ghost layer included) and state as the writers read them: the reference's arrays.
+struct TecplotView {
+    int nDims{1}, nX{1}, nY{1}, nZ{1}, nXAll{1}, nYAll{1};
+    const double *X{nullptr}, *Y{nullptr}, *Z{nullptr}, *hx{nullptr}, *hy{nullptr}, *hz{nullptr};
+    const double* values{nullptr};  // AoS, 5 per cell, serial index (k nYAll + j) nXAll + i
+    double gamma{1.4}, molarMass{0.0}, universalGasConstant{8.3144598}, time{0.0};
+    const double* U(int i, int j, int k) const { return values + (((size_t)k * nYAll + j) * nXAll + i) * 5; }
+    double cx(int i) const { return X ? X[i] : 0.0; }
+    double cy(int j) const { return nDims > 1 ? Y[j] : 0.0; }
+    double cz(int k) const { return nDims > 2 ? Z[k] : 0.0; }
+};
+
+// utility/Slices.h: a line or plane of cells; -1 marks a free index
+struct Slice {
+    int nDims{0}, i{-1}, j{-1}, k{-1};
+    Slice() = default;
+    Slice(int _i, int _j, int _k) : i(_i), j(_j), k(_k) { nDims = (i == -1) + (j == -1) + (k == -1); }
+};
+
+// Kernel::SaveSolutionToTecplot (kernel.h:1081-1416) as one rank writes it
+inline void WriteTecplotSolution(const std::string& fname, const TecplotView& v) {
+    std::ofstream ofs(fname);
+    if (!ofs) throw std::runtime_error("cannot open " + fname);
+    ofs << std::scientific;
+    const int gx = 1, gy = v.nDims > 1 ? 1 : 0, gz = v.nDims > 2 ? 1 : 0;
+    const double g1 = v.gamma - 1.0;
+    if (v.nDims == 1) {
+        ofs << "VARIABLES = \"X\" \"ro\" \"u\" \"v\" \"w\" \"P\" \"e\" " << std::endl;
+        for (int i = gx; i < gx + v.nX; i++) {
+            const double* U = v.U(i, 0, 0);
+            const double ro = U[0], u = U[1] / ro, vv = U[2] / ro, w = U[3] / ro;
+            const double e = U[4] / ro - 0.5 * (u * u + vv * vv + w * w);
+            const double P = g1 * ro * e;
+            ofs << v.cx(i) << " " << ro << " " << u << " " << vv << " " << w << " " << P << " " << e << " " << std::endl;
+        }
+        return;
+    }
+    const int nd = v.nDims;
+    const long long nodes = (long long)(v.nX + 1) * (v.nY + 1) * (nd > 2 ? v.nZ + 1 : 1);
+    const long long cells = (long long)v.nX * v.nY * (nd > 2 ? v.nZ : 1);
+    ofs << (nd == 2 ? "VARIABLES = \"X\" \"Y\" " : "VARIABLES = \"X\" \"Y\" \"Z\" ");
+    ofs << "\"Rho\" \"u\" \"v\" \"w\" \"P\" \"e\"\"T\"" << std::endl;
+    ofs << "ZONE T=\"" << v.time << "\"" << std::endl;
+    ofs << "N=" << nodes << ", E=" << cells << ", F=FEBLOCK, ET=" << (nd == 2 ? "QUADRILATERAL" : "BRICK") << std::endl;
+    ofs << (nd == 2 ? "VARLOCATION = (NODAL, NODAL" : "VARLOCATION = (NODAL, NODAL, NODAL");
+    for (int q = 0; q < 7; q++) ofs << (q == 6 ? ", CELLCENTERED)" : ", CELLCENTERED");
+    ofs << std::endl;
+    const int k0 = gz, k1 = gz + (nd > 2 ? v.nZ : 0);
+    for (int axis = 0; axis < nd; axis++)
+        for (int k = k0; k <= k1; k++)
+            for (int j = gy; j <= gy + v.nY; j++)
+                for (int i = gx; i <= gx + v.nX; i++) {
+                    if (axis == 0) ofs << v.X[i] - 0.5 * v.hx[i];
+                    else if (axis == 1) ofs << v.Y[j] - 0.5 * v.hy[j];
+                    else ofs << v.Z[k] - 0.5 * v.hz[k];
+                    ofs << std::endl;
+                }
+    // cell-centred blocks: rho, u, v, w, P, e, T, each over k, j, i (i fastest)
+    const size_t nc = (size_t)cells;
+    std::vector<double> q[7];
+    for (auto& a : q) a.resize(nc);
+    const double cond = g1 * v.molarMass / v.universalGasConstant;
+    size_t c = 0;
+    for (int k = gz; k < gz + (nd > 2 ? v.nZ : 1); k++)
+        for (int j = gy; j < gy + v.nY; j++)
+            for (int i = gx; i < gx + v.nX; i++, c++) {
+                const double* U = v.U(i, j, k);
+                const double rho = U[0];
+                double u, vv, w, e, P;
+                if (nd == 2) {  // kernel.h:1205-1225
+                    u = U[1] / rho, vv = U[2] / rho, w = U[3] / rho;
+                    e = U[4] / rho;
+                    e = e - 0.5 * (u * u + vv * vv + w * w);
+                    P = g1 * (rho * e);
+                } else {  // kernel.h:1339-1360
+                    const double ke = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / rho;
+                    u = U[1] / rho, vv = U[2] / rho, w = U[3] / rho;
+                    e = U[4] - ke;
+                    P = g1 * e;
+                    e /= rho;
+                }
+                q[0][c] = rho, q[1][c] = u, q[2][c] = vv, q[3][c] = w, q[4][c] = P, q[5][c] = e, q[6][c] = cond * e;
+            }
+    for (auto& a : q)
+        for (double x : a) ofs << x << '\n';
+    // connectivity, Tecplot node numbers from 1
+    for (int k = gz; k < gz + (nd > 2 ? v.nZ : 1); k++)
+        for (int j = gy; j < gy + v.nY; j++) {
+            if (nd == 2) {
+                int bl = (v.nX + 1) * (j - gy) + 1;
+                for (int i = 0; i < v.nX; i++, bl++) {
+                    const int br = bl + 1, ul = bl + (v.nX + 1), ur = ul + 1;
+                    ofs << bl << ' ' << ul << ' ' << ur << ' ' << br << std::endl;
+                }
+            } else {
+                int fbl = (v.nX + 1) * (v.nY + 1) * (k - gz) + (v.nX + 1) * (j - gy) + 1;
+                for (int i = 0; i < v.nX; i++, fbl++) {
+                    const int fbr = fbl + 1, ful = fbl + (v.nX + 1), fur = ful + 1;
+                    const int bbl = fbl + (v.nX + 1) * (v.nY + 1), bbr = bbl + 1, bul = bbl + (v.nX + 1), bur = bul + 1;
+                    ofs << bbr << ' ' << bur << ' ' << bul << ' ' << bbl << ' ' << fbr << ' ' << fur << ' ' << ful << ' '
+                        << fbl << std::endl;
+                }
+            }
+        }
+}
+
+// Kernel::Save2DSliceToTecplot (kernel.h:1416-1511, with_temperature = false) and
+// Kernel::SaveSliceToTecplot (kernel.h:1513-1632, with_temperature = true): the cells with i == I
+// (j == J, k == K) unless that index is -1, visited i, then j, then k (k fastest)
+inline void WriteTecplotSlice(const std::string& fname, const TecplotView& v, int I, int J, int K,
+                              bool with_temperature) {
+    std::ofstream ofs(fname);
+    if (!ofs) throw std::runtime_error("cannot open " + fname);
+    ofs << std::scientific;
+    ofs << "VARIABLES = ";
+    if (I == -1) ofs << "\"X\" ";
+    if (J == -1) ofs << "\"Y\" ";
+    if (K == -1) ofs << "\"Z\" ";
+    ofs << "\"ro\" \"u\" \"v\" \"w\" \"P\" \"e\" ";
+    if (with_temperature) {
+        ofs << "\"T\" ";
+        if (I != -1) ofs << "\"x_cnst\" ";
+        if (J != -1) ofs << "\"y_cnst\" ";
+        if (K != -1) ofs << "\"z_cnst\" ";
+    }
+    ofs << std::endl;
+    ofs << "ZONE T=\"1\"" << std::endl;
+    ofs << "I=" << (I == -1 ? v.nX : 1) << "J=" << (J == -1 ? v.nY : 1) << "K=" << (K == -1 ? v.nZ : 1);
+    ofs << "F=POINT\n";
+    ofs << (with_temperature ? "DT=(SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE)\n"
+                             : "DT=(SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE)\n");
+    const int gy = v.nDims > 1 ? 1 : 0, gz = v.nDims > 2 ? 1 : 0;
+    const double g1 = v.gamma - 1.0;
+    for (int i = 1; i <= v.nX; i++) {
+        if (I != -1 && i != I) continue;
+        for (int j = gy; j < gy + (v.nDims > 1 ? v.nY : 1); j++) {
+            if (J != -1 && j != J) continue;
+            for (int k = gz; k < gz + (v.nDims > 2 ? v.nZ : 1); k++) {
+                if (K != -1 && k != K) continue;
+                const double x = v.cx(i), y = v.cy(j), z = v.cz(k);
+                const double* U = v.U(i, j, k);
+                const double ro = U[0], u = U[1] / ro, vv = U[2] / ro, w = U[3] / ro;
+                const double e = U[4] / ro - 0.5 * (u * u + vv * vv + w * w);
+                const double P = g1 * ro * e;
+                if (I == -1) ofs << x << " ";
+                if (J == -1) ofs << y << " ";
+                if (K == -1) ofs << z << " ";
+                ofs << ro << " " << u << " " << vv << " " << w << " " << P << " " << e << " ";
+                if (with_temperature) {
+                    const double T = g1 * v.molarMass * e / v.universalGasConstant;
+                    ofs << T << " ";
+                    if (I != -1) ofs << x << " ";
+                    if (J != -1) ofs << y << " ";
+                    if (K != -1) ofs << z << " ";
+                }
+                ofs << std::endl;
+            }
+        }
     }
 }
 
@@ -350,7 +631,7 @@ class HipExplicitRungeKuttaFVM {
         if (ctx_) ts_destroy(ctx_), ctx_ = nullptr;
         int rc = ts_create(&c, &ctx_);
         if (rc != TS_OK) {
-            std::string msg = ctx_ ? ts_last_error(ctx_) : "invalid configuration";
+            std::string msg = ts_last_error(ctx_);  // with no context: this thread's create error
             if (ctx_) ts_destroy(ctx_), ctx_ = nullptr;
             throw std::runtime_error("ts_create failed (" + std::to_string(rc) + "): " + msg);
         }

@@ -81,7 +81,7 @@ def test_sod_reference_config():
 def test_venkat_on_stretched_grids(nd, stages):
     """Limited linear reconstruction with the Venkatakrishnan limiter on geometrically stretched
     axes: eps^2 = (K (hx hy hz)^(1/nd))^3 of each cell (Limiters.h:93-96), Roe with Eps = 0.05"""
-    n = {1: (120, 1, 1), 2: (70, 44, 1), 3: (37, 24, 18)}[nd]
+    n = {1: (120, 1, 1), 2: (70, 44, 1), 3: (38, 24, 18)}[nd]
     q = (1.04, 1.06, 1.05)
     geo = [util.stretched(n[a], 1.0, q[a]) if a < nd else (None, None) for a in range(3)]
     cfg = capi.make_config(ndims=nd, nx=n[0], ny=n[1], nz=n[2], cfl=0.4, eps=0.05, riemann=capi.RIEMANN_ROE,

@@ -140,6 +140,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
     // the z-flux below this plane for the previous row is still in lds_h[w] when it is read
     R lamzc = R(0), Szc = R(0);
+    R Szb = R(0);  // z-face area of the pending row (its y size differs from the current row's)
     bool prev_out = false;
     const int nit = yend - y0 + 2;
     for (int t = 0; t < nit; t++) {
@@ -175,6 +176,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         // which lds_h is written again (two barriers per row instead of three)
         if (out_plane && prev_out) {
             lzb = lamzc;
+            Szb = Szc;
             lzt = lds_l[w + 1][lane];
             if constexpr (VIS) vzt = lds_vz[w + 1][lane];
             if constexpr (MT::flux) {
@@ -277,8 +279,8 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                         sr = sr + viscous_sr(double(GL[0]), double(GR[0]), g.gamma, a.v.mu, double(Sy), hx * g.h[1][rc] * hz);
                     {
 #pragma clang fp contract(off)
-                        sr = sr + double(Sz * lzb);
-                        sr = sr + double(Sz * lzt);
+                        sr = sr + double(Szb * lzb);
+                        sr = sr + double(Szb * lzt);
                         if constexpr (VIS) sr = sr + vzt / (hx * g.h[1][rc] * hz);
                     }
                     if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc] * hz, sr));

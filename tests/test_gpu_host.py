@@ -263,3 +263,95 @@ def test_poiseuille_driver_matches_oracle_and_centre_velocity(tmp_path, init, rk
     assert np.all(util.normwise_rel(a, b) <= 1e-10)
     mid = Ud[0, nya // 2 - 1:nya // 2 + 1, nxa // 2]
     assert abs(np.mean(mid[:, 1] / mid[:, 0]) / uc - 1) < 0.01
+
+
+WRITER = r'''
+#include <cstdio>
+#include <vector>
+#include "ts_kernel.hpp"
+// writes the Tecplot file of a 1-D state: argv = nx time grid.bin state.bin out.dat
+int main(int argc, char** argv) {
+    int nx = std::atoi(argv[1]);
+    double t = std::atof(argv[2]);
+    std::vector<double> g(2 * (nx + 2)), U(5 * (nx + 2));
+    FILE* f = std::fopen(argv[3], "rb"); if (std::fread(g.data(), 8, g.size(), f) != g.size()) return 1; std::fclose(f);
+    f = std::fopen(argv[4], "rb"); if (std::fread(U.data(), 8, U.size(), f) != U.size()) return 1; std::fclose(f);
+    tsh::TecplotView v;
+    v.nDims = 1; v.nX = nx; v.nXAll = nx + 2;
+    v.X = g.data(); v.hx = g.data() + nx + 2; v.values = U.data(); v.gamma = 1.4; v.time = t;
+    tsh::WriteTecplotSolution(argv[5], v);
+    return 0;
+}
+'''
+
+
+def _tecplot_of(tmp_path, name, X, H, U, t):
+    """the host writer fed a state computed elsewhere (here: the oracle's)"""
+    src, exe = tmp_path / "writer.cpp", tmp_path / "writer"
+    if not exe.exists():
+        src.write_text(WRITER)
+        pkg = os.path.join(ROOT, "turbostructred_amd")
+        subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "turbostructred_amd", "host"), str(src),
+                        "-o", str(exe), "-L", pkg, "-lts_fvm", f"-Wl,-rpath,{pkg}"], check=True)
+    nx = len(X) - 2
+    np.concatenate([X, H]).astype(np.float64).tofile(tmp_path / "g.bin")
+    np.ascontiguousarray(U, dtype=np.float64).ravel().tofile(tmp_path / "u.bin")
+    out = tmp_path / name
+    subprocess.run([str(exe), str(nx), repr(t), str(tmp_path / "g.bin"), str(tmp_path / "u.bin"), str(out)], check=True)
+    return out.read_text()
+
+
+def _same_tecplot(a, b, rel=1e-5, floor=1e-12):
+    """byte-identical, except that a printed 6-digit value may fall on the other side of a
+    rounding boundary, or be a quantity that is zero up to round-off (a velocity of 1e-14 in the
+    undisturbed state): such lines must agree numerically to the printed precision, with an
+    absolute floor for the round-off values"""
+    la, lb = a.split("\n"), b.split("\n")
+    assert len(la) == len(lb)
+    differ = 0
+    for x, y in zip(la, lb):
+        if x == y:
+            continue
+        differ += 1
+        fx, fy = np.array(x.split(), dtype=float), np.array(y.split(), dtype=float)
+        np.testing.assert_allclose(fx, fy, rtol=rel, atol=floor)
+    return differ
+
+
+def test_reference_sod_driver_as_configured(tmp_path):
+    """shock_tube_driver --reference-sod runs RunSODTestRoe1D (tests.h:13-80) exactly: compressed
+    100-cell grid, Linear2psLim<limVenkatar>, Roe Eps 0.05, CFL 0.5, forward Euler, MaxTime 0.25
+    with SaveSolutionTime 0.25.  Against the oracle: step count, time and state at 1e-12.  Its
+    Tecplot files (init.dat, init2.dat after the .sol save/load, and the snapshot
+    "solution, t = 0.25.dat") equal the same writer fed the oracle's states."""
+    out = tmp_path / "out.bin"
+    print("driver: --reference-sod", flush=True)
+    r = subprocess.run([DRIVER, "--reference-sod", str(tmp_path), str(out)], check=True, timeout=120,
+                       capture_output=True, text=True)
+    assert "Iteration = 100;" in r.stdout  # ResidualOutputIters = 100
+    raw = open(out, "rb").read()
+    hdr = np.frombuffer(raw[:16], dtype=np.int32)
+    nxa, snaps, iters = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    f = np.frombuffer(raw[16:], dtype=np.float64)
+    time_, X, H = f[0], f[7:7 + nxa].copy(), f[7 + nxa:7 + 2 * nxa].copy()
+    Ug = f[7 + 2 * nxa:].reshape(1, 1, nxa, 5)
+    assert nxa == 102 and snaps == 1 and time_ == 0.25
+    cfg = capi.make_config(ndims=1, nx=100, cfl=0.5, eps=0.05, riemann=capi.RIEMANN_ROE, recon=capi.RECON_LINEAR_LIM,
+                           limiter=capi.LIMITER_VENKAT, coord=[X, None, None], h=[H, None, None])
+    left = X < 0.5  # tests.h:85, at the cell centres of the compressed grid (ghosts too: ignored)
+    U0 = util.prim2cons(np.where(left, 1.0, 0.125), 0 * X, 0 * X, 0 * X, np.where(left, 1.0, 0.1), 1.4)
+    U0 = U0.reshape(1, 1, nxa, 5)
+    U0[0, 0, 0] = U0[0, 0, -1] = 0.0  # the host mirror's ghost entries before the first download
+    o = pyoracle.Oracle(cfg)
+    o.set_state(U0)
+    info = o.step(1 << 20, 0.25, 0.25)
+    assert info.time == 0.25 and info.steps_done == iters
+    err = util.normwise_rel(capi.interior(cfg, Ug), capi.interior(cfg, o.get_state()))
+    assert np.all(err <= 1e-12), err
+    init = _tecplot_of(tmp_path, "o_init.dat", X, H, U0, 0.0)
+    assert (tmp_path / "init.dat").read_text() == init
+    assert (tmp_path / "init2.dat").read_text() == init
+    fin = _tecplot_of(tmp_path, "o_fin.dat", X, H, o.get_state(), 0.25)
+    n = _same_tecplot((tmp_path / "solution, t = 0.25.dat").read_text(), fin)
+    print(f"snapshot lines differing in the last printed digit: {n}")
+    assert n <= 10  # of 103 lines: the digits at rounding boundaries and the round-off velocities

@@ -100,10 +100,11 @@ __device__ inline double fsqrt(double x) {
 }
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// 2^600 (double) / 2^60 (float): lifts a divisor below 1e-30 into the normal range exactly
-template <typename R> __device__ inline R big_scale();
-template <> __device__ inline double big_scale<double>() { return 0x1p600; }
-template <> __device__ inline float big_scale<float>() { return 0x1p60f; }
+// smallest divisor the limiter uses: well inside the normal range, where the reciprocal
+// estimate and its Newton steps are accurate
+template <typename R> __device__ inline R tiny_divisor();
+template <> __device__ inline double tiny_divisor<double>() { return 1e-290; }
+template <> __device__ inline float tiny_divisor<float>() { return 1e-35f; }
 
 template <typename R> __device__ inline R tiny_spread();
 template <> __device__ inline double tiny_spread<double>() { return 3.0 * 2.2250738585072014e-308; }
@@ -162,28 +163,19 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                 if (!(umax - umin < tiny_spread<R>())) {
                     if constexpr (LIM == LIM_BJ) {
                         // Barth-Jespersen (Limiters.h:80-82).  Along each axis the two face
-                        // projections are +-d_a, one bounded by umax and the other by umin, so
-                        // min over faces of (u_m - u_i)/d2 = min(umax - c, c - umin) / max_a |d_a|.
+                        // projections are +-d_a (d_a = g_a h_a / 2), one bounded by umax and the
+                        // other by umin, so min over faces of min(1, (u_m - u_i)/d2) is
+                        // min(umax - c, c - umin) / max_a |d_a| when that is below 1.  The
+                        // projection is taken as g_a h_a / 2 itself rather than the reference's
+                        // (c + g.p) - c: the two differ by an ulp of c, and so do the face values.
                         R m = __builtin_fmin(umax - c[v], c[v] - umin);
                         R dmax = R(0);
 #pragma unroll
-                        for (int a = 0; a < ND; a++) {
-                            R d2;
-                            {
-#pragma clang fp contract(off)
-                                // projection (v + g.p) - v, rounded as Linear2psLim.h:77-89
-                                R fv = c[v] + g[a] * geo.hh[a];
-                                d2 = fv - c[v];
-                            }
-                            dmax = __builtin_fmax(dmax, __builtin_fabs(d2));
-                        }
-                        if (m < dmax) {
-                            // the divisor can be subnormal (a variable that is zero up to
-                            // round-off), where the reciprocal estimate fails: both operands
-                            // are then scaled by an exact power of two (0 <= m < dmax)
-                            const R s = (dmax > R(1e-30)) ? R(1) : big_scale<R>();
-                            lim = fdiv(m * s, dmax * s);
-                        }
+                        for (int a = 0; a < ND; a++) dmax = __builtin_fmax(dmax, __builtin_fabs(g[a] * geo.hh[a]));
+                        // a divisor below the floor (a variable constant up to round-off) is
+                        // raised to it: the reciprocal estimate stays in range, the quotient stays
+                        // below 1, and the face values move by less than the floor
+                        if (m < dmax) lim = fdiv(m, dmax > tiny_divisor<R>() ? dmax : tiny_divisor<R>());
                     } else {
                         R bestA = R(1), bestD = R(1);
                         R inv_un = frcp(R(0.5) * (umax - umin));
@@ -216,14 +208,19 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
             }
 #pragma unroll
             for (int a = 0; a < ND; a++) {
-                R dl = g[a] * (-geo.hh[a]);
-                R dr = g[a] * geo.hh[a];
+                R d = g[a] * geo.hh[a];
                 if constexpr (RECON == RC_LINLIM) {  // Linear2psLim.h:17-21
-                    face[2 * a][v] = c[v] + lim * dl;
-                    face[2 * a + 1][v] = c[v] + lim * dr;
+                    if constexpr (LIM == LIM_BJ) {
+                        const R t = lim * d;
+                        face[2 * a][v] = c[v] - t;
+                        face[2 * a + 1][v] = c[v] + t;
+                    } else {
+                        face[2 * a][v] = c[v] + lim * (g[a] * (-geo.hh[a]));
+                        face[2 * a + 1][v] = c[v] + lim * d;
+                    }
                 } else {  // Linear2PointsStencil.h:26-30
-                    face[2 * a][v] = c[v] + dl;
-                    face[2 * a + 1][v] = c[v] + dr;
+                    face[2 * a][v] = c[v] - d;
+                    face[2 * a + 1][v] = c[v] + d;
                 }
             }
         }

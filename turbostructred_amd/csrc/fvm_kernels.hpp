@@ -508,6 +508,9 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         R (&gbot)[NV][64] = carry[wl][0];
         R (&dfx)[NV][64] = carry[wl][1];
         int sm = 0, s0 = 1, sp = 2;  // slots of rows rc-1, rc, rc+1
+        // x-neighbours are read from the same LDS rows at lane -+ 1 (the end lanes, halo lanes
+        // whose results are discarded, read themselves): LDS reads instead of VALU lane shifts
+        const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
         double srp = 0.0;       // spectral radius of the pending row without its top face
         R lam_gb = R(0);
         {
@@ -553,8 +556,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     U0[v] = ring[wl][s0][v][lane];
-                    nb[0][v] = dpp_from_left(U0[v]);
-                    nb[1][v] = dpp_from_right(U0[v]);
+                    nb[0][v] = ring[wl][s0][v][lm];
+                    nb[1][v] = ring[wl][s0][v][lp];
                     nb[2][v] = ring[wl][sm][v][lane];
                     nb[3][v] = ring[wl][sp][v][lane];
                 }
@@ -612,11 +615,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                     for (int v = 0; v < NV; v++) gbot[v][lane] = G[v];
                 }
                 if (rc < yend) {
-                    // x-faces of row rc: each lane evaluates its minus face
+                    // x-faces of row rc: each lane evaluates its minus face.  The plus-face
+                    // states of the left neighbours, then the fluxes and eigenvalues of the right
+                    // neighbours, pass through the LDS slot of row rc-1, free after its finalize
+                    // (LDS operations of one wavefront complete in order).
+                    R (&xch)[NV][64] = ring[wl][sm];
                     R UL[NV], UR[NV];
 #pragma unroll
+                    for (int v = 0; v < NV; v++) xch[v][lane] = face[1][v];
+#pragma unroll
                     for (int v = 0; v < NV; v++) {
-                        UL[v] = dpp_from_left(face[1][v]);
+                        UL[v] = xch[v][lm];
                         UR[v] = face[0][v];
                     }
                     if (!g.periodic[0]) {
@@ -634,8 +643,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                             for (int v = 1; v < NV; v++) F[v] -= R(ld_plane(a.v.fv[0], a.v.nface[0], v - 1, fi));
                         }
                     }
+                    if constexpr (MT::flux) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) xch[v][lane] = F[v];
+#pragma unroll
+                        for (int v = 0; v < NV; v++) dfx[v][lane] = -((xch[v][lp] - F[v]) * Sx);
+                    }
                     if constexpr (MT::sr) {
-                        R lam_r = dpp_from_right(lamx);
+                        xch[0][lane] = lamx;
+                        const R lam_r = xch[0][lp];
                         {
 #pragma clang fp contract(off)
                             srp = double(Sx * lamx) + double(Sx * lam_r);
@@ -650,13 +666,6 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                         }
                         if (ghost_x_lane)  // right ghost column (ERK:1225)
                             block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy, double(Sx * lamx)));
-                    }
-                    if constexpr (MT::flux) {
-#pragma unroll
-                        for (int v = 0; v < NV; v++) {
-                            R Fr = dpp_from_right(F[v]);
-                            dfx[v][lane] = -((Fr - F[v]) * Sx);
-                        }
                     }
                 }
             }

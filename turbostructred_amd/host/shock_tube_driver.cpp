@@ -1,10 +1,19 @@
-// shock_tube_driver.cpp -- a 1-D shock-tube run written the way the reference's own drivers are
+// shock_tube_driver.cpp -- 1-D shock-tube runs written the way the reference's own drivers are
 // (src/Tests/tests.h:13-80 RunSODTestRoe1D, src/Tests/1DTests/ToroTests.h), against the
 // HipExplicitRungeKuttaFVM host class: KernelConfiguration, CreateKernel, Init,
 // SetInitialConditions, Run.
 //
+// usage: shock_tube_driver --reference-sod out_dir out.bin
+//   RunSODTestRoe1D exactly as the reference configures it (tests.h:13-80): 100 cells on the
+//   two-block compressed grid, Natural boundaries, gamma 1.4, CFL 0.5, forward Euler, Eps 0.05,
+//   Linear2psLim<limVenkatar>, Roe-Pike, MaxTime 0.25, SaveSolutionTime 0.25,
+//   ResidualOutputIters 100; init.dat, sol.sol (saved and loaded back), init2.dat and the
+//   snapshot "solution, t = 0.25.dat" are written into out_dir.
+//
 // usage: shock_tube_driver rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress
 //                          save_time residual_norms out.bin [save.sol load.sol [probe_dir]]
+//   a parametrised shock tube with CFL 0.4, Eps 0, the Barth-Jespersen limiter for recon 3 and
+//   no snapshot files (the parity tests' cases)
 //   recon / riemann: the reference's enum values; rk: 1 or 3; compress: 1 = the two-block
 //   geometric grid of RunSODTestRoe1D (q = 1/1.05 then 1.05, node at 0.5 L), 0 = uniform;
 //   save_time > 0 clamps dt onto its multiples (SaveSolutionTime); save.sol / load.sol ("-" for
@@ -24,7 +33,83 @@
 
 using namespace tsh;
 
+static void write_out(const HipExplicitRungeKuttaFVM& k, int snapshots, const char* out) {
+    FILE* f = std::fopen(out, "wb");
+    if (!f) throw std::runtime_error("cannot open output file");
+    int hdr[4] = {k.nXAll, snapshots, k.stepInfo.Iteration, 0};
+    double st[7] = {k.stepInfo.Time, k.stepInfo.TimeStep};
+    for (int v = 0; v < 5; v++) st[2 + v] = k.stepInfo.Residual[v];
+    std::fwrite(hdr, sizeof(int), 4, f);
+    std::fwrite(st, sizeof(double), 7, f);
+    std::fwrite(k.CoordinateX.data(), sizeof(double), k.nXAll, f);
+    std::fwrite(k.hx.data(), sizeof(double), k.nXAll, f);
+    std::fwrite(&k.values[0], sizeof(double), k.values.size(), f);
+    std::fclose(f);
+}
+
+// RunSODTestRoe1D (tests.h:13-80) as the reference configures and runs it
+static int reference_sod(const std::string& dir, const char* out) {
+    KernelConfiguration conf;
+    conf.nDims = 1;
+    conf.LX = 1.0;
+    conf.nX = 100;
+    conf.isPeriodicX = false;
+    conf.DummyLayerSize = 1;
+    BlockNode nleft, ncenter;  // grid compression (tests.h:25-32)
+    nleft.N_cells = conf.nX / 2;
+    nleft.q_com = 1.0 / 1.05;
+    ncenter.pos = 0.5 * conf.LX;
+    ncenter.N_cells = conf.nX - nleft.N_cells;
+    ncenter.q_com = 1.0 / nleft.q_com;
+    conf.CompressionX[0] = nleft;
+    conf.CompressionX.push_back(ncenter);
+    conf.MyConditions[1] = BoundaryConditionConfiguration(BoundaryConditionType::Natural);
+    conf.xLeftBoundary.SetMarker(1);
+    conf.xRightBoundary.SetMarker(1);
+    conf.Gamma = 1.4;
+    conf.IsViscousFlow = false;
+    conf.methodConfiguration.CFL = 0.5;
+    conf.methodConfiguration.RungeKuttaOrder = 1;
+    conf.methodConfiguration.Eps = 0.05;
+    conf.methodConfiguration.ReconstructionType = Reconstruction::Linear2psLim;
+    conf.methodConfiguration.GeneralLimitter = LimiterType::Venkatakrishnan;  // Linear2psLim<limVenkatar>
+    conf.methodConfiguration.RiemannProblemSolver = RPSolver::RoePikeSolver;
+    conf.MaxTime = 0.25;
+    conf.MaxIteration = 1000000;
+    conf.SaveSolutionTime = 0.25;
+    conf.SaveSolutionIters = 0;
+    conf.ResidualOutputIters = 100;
+    auto kernel = CreateKernel(conf);
+    kernel->Init(conf);
+    kernel->OutputDirectory = dir;
+    const double roL = 1.0, PL = 1.0, uL = 0, roR = 0.125, PR = 0.1, uR = 0, x0 = 0.5;
+    kernel->SetInitialConditions([&](const Vector& r) {
+        const bool left = r.x < x0;
+        const double ro = left ? roL : roR, u = left ? uL : uR, p = left ? PL : PR;
+        return std::vector<double>{ro, ro * u, 0.0, 0.0, p / (conf.Gamma - 1.0) + 0.5 * ro * u * u};
+    });
+    kernel->SaveSolutionToTecplot(dir + "/init.dat");
+    kernel->SaveSolution(dir + "/sol.sol");
+    kernel->LoadSolution(dir + "/sol.sol");
+    kernel->SaveSolutionToTecplot(dir + "/init2.dat");
+    int snapshots = 0;
+    kernel->OnSolutionSnapshot = [&](HipExplicitRungeKuttaFVM&) { snapshots++; };
+    kernel->Run();
+    kernel->SyncValues();
+    write_out(*kernel, snapshots, out);
+    std::printf("steps %d time %.17g snapshots %d\n", kernel->stepInfo.Iteration, kernel->stepInfo.Time, snapshots);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 4 && std::string(argv[1]) == "--reference-sod") {
+        try {
+            return reference_sod(argv[2], argv[3]);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc != 17 && argc != 19 && argc != 20) {
         std::fprintf(stderr,
                      "usage: %s rhoL uL pL rhoR uR pR x0 t_end nx recon riemann rk compress save_time norms out.bin"
@@ -76,6 +161,7 @@ int main(int argc, char** argv) {
         auto kernel = CreateKernel(conf);
         kernel->Init(conf);
         kernel->ComputeResidualNorms = norms != 0;
+        kernel->WriteSnapshotFiles = false;
         const double g1 = conf.Gamma - 1.0;
         kernel->SetInitialConditions([&](const Vector& r) {
             const bool left = r.x < x0;  // tests.h:85
@@ -110,17 +196,7 @@ int main(int argc, char** argv) {
         kernel->SyncValues();
         if (save_sol != "-") kernel->SaveSolution(save_sol);
 
-        FILE* f = std::fopen(out, "wb");
-        if (!f) throw std::runtime_error("cannot open output file");
-        int hdr[4] = {kernel->nXAll, snapshots, kernel->stepInfo.Iteration, 0};
-        double st[7] = {kernel->stepInfo.Time, kernel->stepInfo.TimeStep};
-        for (int v = 0; v < 5; v++) st[2 + v] = kernel->stepInfo.Residual[v];
-        std::fwrite(hdr, sizeof(int), 4, f);
-        std::fwrite(st, sizeof(double), 7, f);
-        std::fwrite(kernel->CoordinateX.data(), sizeof(double), kernel->nXAll, f);
-        std::fwrite(kernel->hx.data(), sizeof(double), kernel->nXAll, f);
-        std::fwrite(&kernel->values[0], sizeof(double), kernel->values.size(), f);
-        std::fclose(f);
+        write_out(*kernel, snapshots, out);
         std::printf("steps %d time %.17g snapshots %d\n", kernel->stepInfo.Iteration, kernel->stepInfo.Time, snapshots);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

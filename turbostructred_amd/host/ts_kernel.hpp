@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <fstream>
+#include <sstream>
 #include <functional>
 #include <iostream>
 #include <limits>
@@ -110,6 +111,10 @@ struct KernelConfiguration {
     double SaveSolutionTime{0};
     double SaveSliceTime{0};
     int SaveSolutionIters{0};
+    int SaveSliceIters{0};
+    int SaveBinarySolIters{0};
+    double MolarMass{0.0};                 // temperature column of the Tecplot writers
+    double UniversalGasConstant{8.3144598};
     int ResidualOutputIters{0};
     int ChangeHistoryFileIters{5000000};
     // history.dat every SaveHistoryFileIters iterations; the reference's default is 10, here it
@@ -493,6 +498,14 @@ class HipExplicitRungeKuttaFVM {
     double MaxTime{1.0};
     double SaveSolutionTime{0};
     double SaveSliceTime{0};
+    int SaveSolutionIters{0};
+    int SaveSliceIters{0};
+    int SaveBinarySolIters{0};
+    double Gamma{1.4}, MolarMass{0.0}, UniversalGasConstant{8.3144598};
+    std::vector<Slice> slices;  // written by Run at the slice snapshot times / iterations
+    // Run writes the reference's Tecplot snapshot files ("solution, t = ....dat", "slice<i>,T=....dat",
+    // kernel.h:1723-1797) into OutputDirectory; the snapshot hooks below are called as well
+    bool WriteSnapshotFiles{true};
     bool ComputeResidualNorms{false};
     int ResidualOutputIters{0};
     int SaveHistoryFileIters{0};
@@ -624,6 +637,12 @@ class HipExplicitRungeKuttaFVM {
         NextHistoryFileIters = ChangeHistoryFileIters;  // kernel.h:252-254
         SaveSolutionTime = config.SaveSolutionTime;
         SaveSliceTime = config.SaveSliceTime;
+        SaveSolutionIters = config.SaveSolutionIters;
+        SaveSliceIters = config.SaveSliceIters;
+        SaveBinarySolIters = config.SaveBinarySolIters;
+        Gamma = config.Gamma;
+        MolarMass = config.MolarMass;  // kernel.h:214-215
+        UniversalGasConstant = config.UniversalGasConstant;
         stepInfo = StepInfo();
         stepInfo.NextSolutionSnapshotTime = SaveSolutionTime;  // kernel.h:261-262
         stepInfo.NextSliceSnapshotTime = SaveSliceTime;
@@ -653,6 +672,55 @@ class HipExplicitRungeKuttaFVM {
                     for (int v = 0; v < nVariables; v++) values[s + v] = U[v];
                 }
         UploadValues();
+    }
+
+    // Kernel::SetInitialConditions with cell averages (kernel.h:149-175): the quadrature's
+    // integral of initF over each interior cell divided by the cell volume hx hy hz
+    void SetInitialConditions(const std::function<std::vector<double>(const Vector& r)>& initF,
+                              NumericQuadrature& Int) {
+        const int g = 1;
+        for (int k = 0; k < nZ; k++)
+            for (int j = 0; j < nY; j++)
+                for (int i = 0; i < nX; i++) {
+                    const int I = i + g, J = nDims > 1 ? j + g : 0, K = nDims > 2 ? k + g : 0;
+                    const double x = CoordinateX[I], y = nDims > 1 ? CoordinateY[J] : 0.0;
+                    const double z = nDims > 2 ? CoordinateZ[K] : 0.0;
+                    const double hxc = hx[I], hyc = nDims > 1 ? hy[J] : 1.0, hzc = nDims > 2 ? hz[K] : 1.0;
+                    const double cellV = hxc * hyc * hzc;
+                    const std::valarray<double> avg =
+                        Int.IntegrateGroap(x, y, z, hxc, hyc, hzc, initF, (size_t)nVariables) / cellV;
+                    const size_t s = (((size_t)K * nYAll + J) * nXAll + I) * nVariables;
+                    for (int v = 0; v < nVariables; v++) values[s + v] = avg[v];
+                }
+        UploadValues();
+    }
+
+    // the grid and the host mirror as the Tecplot writers read them
+    TecplotView View() const {
+        TecplotView v;
+        v.nDims = nDims, v.nX = nX, v.nY = nY, v.nZ = nZ, v.nXAll = nXAll, v.nYAll = nYAll;
+        v.X = CoordinateX.data(), v.hx = hx.data();
+        if (nDims > 1) v.Y = CoordinateY.data(), v.hy = hy.data();
+        if (nDims > 2) v.Z = CoordinateZ.data(), v.hz = hz.data();
+        v.values = &values[0];
+        v.gamma = Gamma, v.molarMass = MolarMass, v.universalGasConstant = UniversalGasConstant;
+        v.time = stepInfo.Time;
+        return v;
+    }
+    // Kernel::SaveSolutionToTecplot (kernel.h:1081-1416): the state comes from the device first
+    void SaveSolutionToTecplot(const std::string& fname) {
+        SyncValues();
+        WriteTecplotSolution(fname, View());
+    }
+    // Kernel::Save2DSliceToTecplot (kernel.h:1416-1511); I/J/K with the ghost offset, -1 = free
+    void Save2DSliceToTecplot(const std::string& fname, int I, int J, int K) {
+        SyncValues();
+        WriteTecplotSlice(fname, View(), I, J, K, false);
+    }
+    // Kernel::SaveSliceToTecplot (kernel.h:1513-1632)
+    void SaveSliceToTecplot(const std::string& fname, const Slice& s) {
+        SyncValues();
+        WriteTecplotSlice(fname, View(), s.i, s.j, s.k, true);
     }
 
     // push the host mirror to the device (after editing `values` by hand)
@@ -692,6 +760,10 @@ class HipExplicitRungeKuttaFVM {
             if (hist) upto(SaveHistoryFileIters);
             if (sensing) upto(SaveSensorRecordIters);
             if (ResidualOutputIters > 0) upto(ResidualOutputIters);
+            if (WriteSnapshotFiles) {
+                upto(SaveSolutionIters);
+                if (!slices.empty()) upto(SaveSliceIters), upto(SaveBinarySolIters);
+            }
             if (hist && NextHistoryFileIters >= it) left = std::min<long long>(left, NextHistoryFileIters + 1 - it);
             if (left < 1) left = 1;
             ts_step_info info{};
@@ -708,6 +780,7 @@ class HipExplicitRungeKuttaFVM {
                     sn->UpdateTimer(stepInfo.Time);
                     sn->Process(*this);
                 }
+            if (WriteSnapshotFiles) IterationSnapshots(now);
             Snapshots();
             if (hist) {
                 if (now > NextHistoryFileIters) {
@@ -834,13 +907,55 @@ class HipExplicitRungeKuttaFVM {
         if (!sol && !slc) return;
         SyncValues();
         if (sol) {
+            if (WriteSnapshotFiles) {  // kernel.h:1762-1777
+                std::stringstream fn;
+                fn << std::fixed;
+                fn.precision(TimePrecision(SaveSolutionTime));
+                fn << "solution, t = " << stepInfo.Time << ".dat";
+                WriteTecplotSolution(OutputDirectory + "/" + fn.str(), View());
+            }
             if (OnSolutionSnapshot) OnSolutionSnapshot(*this);
             stepInfo.NextSolutionSnapshotTime += SaveSolutionTime;
         }
         if (slc) {
+            if (WriteSnapshotFiles)  // kernel.h:1778-1795
+                for (size_t i = 0; i < slices.size(); i++) {
+                    std::stringstream fn;
+                    fn << "slice" << i;
+                    fn << std::fixed;
+                    fn.precision(TimePrecision(SaveSliceTime));
+                    fn << ",T=" << stepInfo.Time << ".dat";
+                    const Slice& s = slices[i];
+                    WriteTecplotSlice(OutputDirectory + "/" + fn.str(), View(), s.i, s.j, s.k, true);
+                }
             if (OnSliceSnapshot) OnSliceSnapshot(*this);
             stepInfo.NextSliceSnapshotTime += SaveSliceTime;
         }
+    }
+    // digits of the snapshot times in file names (kernel.h:1636-1644)
+    static int TimePrecision(double every) { return static_cast<int>(1 - std::floor(std::log10(every))); }
+    // the snapshots every few iterations (kernel.h:1723-1760)
+    void IterationSnapshots(int it) {
+        const bool sol = SaveSolutionIters != 0 && it % SaveSolutionIters == 0;
+        const bool slc = SaveSliceIters != 0 && it % SaveSliceIters == 0 && !slices.empty();
+        const bool bin = SaveBinarySolIters != 0 && it % SaveBinarySolIters == 0 && !slices.empty();
+        if (!sol && !slc && !bin) return;
+        SyncValues();
+        if (sol) {
+            std::stringstream fn;
+            fn << "solution, It = " << it << ".dat";
+            WriteTecplotSolution(OutputDirectory + "/" + fn.str(), View());
+        }
+        if (slc)
+            for (size_t i = 0; i < slices.size(); i++) {
+                std::stringstream fn;
+                fn << "slice" << i << ", It =" << it << ".dat";
+                const Slice& s = slices[i];
+                WriteTecplotSlice(OutputDirectory + "/" + fn.str(), View(), s.i, s.j, s.k, true);
+            }
+        // the reference saves solution.sol once per slice (kernel.h:1752-1759)
+        if (bin)
+            for (size_t i = 0; i < slices.size(); i++) SaveSolution(OutputDirectory + "/solution.sol");
     }
 };
 

@@ -100,6 +100,15 @@ __device__ inline double fsqrt(double x) {
 }
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// one-instruction maximum / minimum in the arithmetic type (v_max_f64 / v_max_f32, no
+// conversion to double for float operands)
+__device__ inline double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ inline float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ inline double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ inline float vmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ inline double vabs(double a) { return __builtin_fabs(a); }
+__device__ inline float vabs(float a) { return __builtin_fabsf(a); }
+
 // smallest divisor the limiter uses: well inside the normal range, where the reciprocal
 // estimate and its Newton steps are accurate
 template <typename R> __device__ inline R tiny_divisor();
@@ -138,6 +147,41 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
         if constexpr (RECON == RC_PWC) {  // PiecewiseConstant.h:15
 #pragma unroll
             for (int f = 0; f < 2 * ND; f++) face[f][v] = c[v];
+        } else if constexpr (RECON == RC_LINLIM && LIM == LIM_BJ) {
+            // Limited linear with Barth-Jespersen (Linear2psLim.h:64-96, Limiters.h:28-67,80-82).
+            // Along each axis the two face projections are +-d_a, d_a = g_a h_a / 2 with the
+            // central slope g_a (Linear2PointsStencil.h:66-97); one is bounded by umax and the
+            // other by umin, so the minimum over the faces of min(1, (u_m - u_i)/d2) is
+            // min(umax - c, c - umin) / max_a |d_a| when that is below 1.  The projection is
+            // formed as (u_+ - u_-) (h_a / (2 (x_+ - x_-))) rather than the reference's
+            // (c + g.p) - c: the two differ by an ulp of c, and so do the face values.  A
+            // variable without spread has d_a = 0 and keeps its cell value, as the reference's
+            // spread test (Limiters.h:50) gives.
+            R d[ND], dmax = R(0);
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                d[a] = (nb[2 * a + 1][v] - nb[2 * a][v]) * (geo.rc[a] * geo.hh[a]);
+                dmax = vmax(dmax, vabs(d[a]));
+            }
+            R umax = c[v], umin = c[v];
+#pragma unroll
+            for (int n = 0; n < 2 * ND; n++) {
+                umax = vmax(umax, nb[n][v]);
+                umin = vmin(umin, nb[n][v]);
+            }
+            const R hi = umax - c[v], lo = c[v] - umin;
+            const R m = vmin(hi, lo);
+            R lim = R(1);
+            // a divisor below the floor (a variable constant up to round-off) is raised to it:
+            // the reciprocal estimate stays in range, the quotient below 1, and the face values
+            // move by less than the floor
+            if (m < dmax) lim = fdiv(m, vmax(dmax, tiny_divisor<R>()));
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                const R t = lim * d[a];
+                face[2 * a][v] = c[v] - t;
+                face[2 * a + 1][v] = c[v] + t;
+            }
         } else {
             R g[ND];
 #pragma unroll
@@ -161,22 +205,7 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
                     umin = __builtin_fmin(umin, nb[n][v]);
                 }
                 if (!(umax - umin < tiny_spread<R>())) {
-                    if constexpr (LIM == LIM_BJ) {
-                        // Barth-Jespersen (Limiters.h:80-82).  Along each axis the two face
-                        // projections are +-d_a (d_a = g_a h_a / 2), one bounded by umax and the
-                        // other by umin, so min over faces of min(1, (u_m - u_i)/d2) is
-                        // min(umax - c, c - umin) / max_a |d_a| when that is below 1.  The
-                        // projection is taken as g_a h_a / 2 itself rather than the reference's
-                        // (c + g.p) - c: the two differ by an ulp of c, and so do the face values.
-                        R m = __builtin_fmin(umax - c[v], c[v] - umin);
-                        R dmax = R(0);
-#pragma unroll
-                        for (int a = 0; a < ND; a++) dmax = __builtin_fmax(dmax, __builtin_fabs(g[a] * geo.hh[a]));
-                        // a divisor below the floor (a variable constant up to round-off) is
-                        // raised to it: the reciprocal estimate stays in range, the quotient stays
-                        // below 1, and the face values move by less than the floor
-                        if (m < dmax) lim = fdiv(m, dmax > tiny_divisor<R>() ? dmax : tiny_divisor<R>());
-                    } else {
+                    {  // Venkatakrishnan (Barth-Jespersen takes the branch above)
                         R bestA = R(1), bestD = R(1);
                         R inv_un = frcp(R(0.5) * (umax - umin));
 #pragma unroll
@@ -209,15 +238,9 @@ __device__ inline void reconstruct(const R (&c)[NV], const R (&nb)[2 * ND][NV],
 #pragma unroll
             for (int a = 0; a < ND; a++) {
                 R d = g[a] * geo.hh[a];
-                if constexpr (RECON == RC_LINLIM) {  // Linear2psLim.h:17-21
-                    if constexpr (LIM == LIM_BJ) {
-                        const R t = lim * d;
-                        face[2 * a][v] = c[v] - t;
-                        face[2 * a + 1][v] = c[v] + t;
-                    } else {
-                        face[2 * a][v] = c[v] + lim * (g[a] * (-geo.hh[a]));
-                        face[2 * a + 1][v] = c[v] + lim * d;
-                    }
+                if constexpr (RECON == RC_LINLIM) {  // Linear2psLim.h:17-21 (Venkatakrishnan)
+                    face[2 * a][v] = c[v] + lim * (g[a] * (-geo.hh[a]));
+                    face[2 * a + 1][v] = c[v] + lim * d;
                 } else {  // Linear2PointsStencil.h:26-30
                     face[2 * a][v] = c[v] - d;
                     face[2 * a + 1][v] = c[v] + d;
@@ -366,7 +389,11 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
     bool supersonic = left ? (S >= R(0)) : (S <= R(0));
     // supersonic side: the star correction is multiplied by zero (and its operands made finite),
     // which gives Fk exactly with two selects instead of five
-    R coef = supersonic ? R(0) : fdiv(m, S - Ss);
+    // m / (S - Ss) and p / m from one reciprocal of m (S - Ss), which is positive on a
+    // subsonic side (S < 0 <= Ss with m = rho (S - u) < 0, or the mirror case)
+    const R dS = S - Ss;
+    const R rmd = frcp(supersonic ? R(1) : m * dS);
+    R coef = supersonic ? R(0) : m * m * rmd;
     R Sc = supersonic ? R(0) : S;
     R Us[NV];
     Us[0] = coef;
@@ -374,7 +401,7 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
     Us[2] = coef * vel[1];
     Us[3] = coef * vel[2];
     Us[1 + D] = coef * Ss;
-    R pm = supersonic ? R(0) : fdiv(p, m);
+    R pm = supersonic ? R(0) : p * dS * rmd;
     Us[4] = coef * (U[4] * ik + (Ss - un) * (Ss + pm));
 #pragma unroll
     for (int i = 0; i < NV; i++) F[i] = Fk[i] + Sc * (Us[i] - U[i]);

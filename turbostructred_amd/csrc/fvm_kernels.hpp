@@ -501,12 +501,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         // The three-row window (rows rc-1, rc, rc+1) and the values carried to the next row live
         // in LDS, one set per wavefront; no other lane reads them, so no barrier is needed.  The
         // window rotates by slot index instead of register moves, and out of the register file
-        // these 25 doubles per lane let three wavefronts share a SIMD.
-        __shared__ R ring[4][3][NV][64];   // [wavefront][slot][variable][lane]
-        __shared__ R carry[4][2][NV][64];  // y-flux through the pending row's bottom face, x residual
+        // these 20 doubles per lane (40 KB per 4-wavefront block in FP64) leave the occupancy to
+        // the register count.
+        __shared__ R ring[4][3][NV][64];  // [wavefront][slot][variable][lane]
+        // residual carried to the pending row: its x part plus S_y times the y-flux through its
+        // bottom face; the finalize subtracts S_y times the flux through its top face
+        __shared__ R carry[4][NV][64];
         const int wl = (threadIdx.x >> 6) & 3;
-        R (&gbot)[NV][64] = carry[wl][0];
-        R (&dfx)[NV][64] = carry[wl][1];
+        R (&cy)[NV][64] = carry[wl];
         int sm = 0, s0 = 1, sp = 2;  // slots of rows rc-1, rc, rc+1
         // x-neighbours are read from the same LDS rows at lane -+ 1 (the end lanes, halo lanes
         // whose results are discarded, read themselves): LDS reads instead of VALU lane shifts
@@ -606,13 +608,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                         R res[NV], Um[NV];
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
-                            res[v] = dfx[v][lane] - (G[v] - gbot[v][lane]) * Sy;
+                            res[v] = cy[v][lane] - G[v] * Sy;
                             Um[v] = ring[wl][sm][v][lane];
                         }
                         finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
                     }
 #pragma unroll
-                    for (int v = 0; v < NV; v++) gbot[v][lane] = G[v];
+                    for (int v = 0; v < NV; v++) cy[v][lane] = G[v] * Sy;
                 }
                 if (rc < yend) {
                     // x-faces of row rc: each lane evaluates its minus face.  The plus-face
@@ -647,7 +649,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 #pragma unroll
                         for (int v = 0; v < NV; v++) xch[v][lane] = F[v];
 #pragma unroll
-                        for (int v = 0; v < NV; v++) dfx[v][lane] = -((xch[v][lp] - F[v]) * Sx);
+                        for (int v = 0; v < NV; v++) cy[v][lane] = cy[v][lane] - (xch[v][lp] - F[v]) * Sx;
                     }
                     if constexpr (MT::sr) {
                         xch[0][lane] = lamx;

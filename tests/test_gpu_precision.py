@@ -33,6 +33,7 @@ def test_shock_tubes_fp32_residual():
         b, ib = solve(fp64_twin(cfg), U, t_end=t_end)
         assert abs(ia.time - t_end) <= 1e-12 * t_end and abs(ib.time - t_end) <= 1e-12 * t_end
         d = np.abs(a - b)
+        print(f"tube n={cfg.nx}: density L1 {d.mean():.3e} Linf {d.max():.3e}")
         assert d.mean() <= 1e-5 and d.max() <= 1e-2, (cfg.nx, d.mean(), d.max())
 
 
@@ -49,6 +50,7 @@ def test_quadrants_fp32_residual(riemann, recon):
     b, ib = solve(fp64_twin(cfg), U, steps=150)
     assert abs(ia.time - ib.time) <= 1e-5 * ib.time
     d = np.abs(a - b)
+    print(f"quadrants riemann={riemann} recon={recon}: density L1 {d.mean():.3e} Linf {d.max():.3e}")
     assert d.mean() <= 1e-6 and d.max() <= 1e-4, (d.mean(), d.max())
 
 
@@ -70,4 +72,5 @@ def test_taylor_green_fp32_residual():
     a, ia = solve(cfg, U, steps=40)
     b, ib = solve(fp64_twin(cfg), U, steps=40)
     d = np.abs(a - b)
+    print(f"taylor-green 48^3: density L1 {d.mean():.3e} Linf {d.max():.3e}")
     assert d.mean() <= 1e-6 and d.max() <= 1e-5, (d.mean(), d.max())

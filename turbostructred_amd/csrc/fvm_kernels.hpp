@@ -4,10 +4,11 @@
 // lanes on each side) and marches along y over `ty` output rows.  In 2-D it keeps a three-row
 // window of the state, and the values carried from one row to the next, in its own LDS slots
 // (no other lane reads them, so there are no barriers), with the next row in flight in
-// registers (prefetched one iteration ahead).  x-neighbours come from the adjacent lanes with
-// DPP wave shifts; every face flux is evaluated once, by the lane on its plus side, and the
-// residual of a cell is accumulated in x-then-y order (ERK:1232,1312) before the stage update
-// is applied and the row is stored with coalesced SoA writes.
+// registers (prefetched one iteration ahead).  In 2-D, x-neighbours are read from the same LDS
+// rows at the adjacent lanes (1-D and 3-D use DPP wave shifts); every face flux is evaluated
+// once, by the lane on its plus side, and the residual of a cell is accumulated from its x
+// faces, then its y faces (ERK:1232,1312) before the stage update is applied and the row is
+// stored with coalesced SoA writes.
 //
 // Modes
 //   MODE_DT  spectral radius of every cell from the face eigenvalues, min(V/SR) over the grid
@@ -98,6 +99,8 @@ struct KArgs {
     const void* lin;   // L(Un) planes (TOut type), MODE_F2
     const double* un;  // U^n planes (SSP stages 2, 3)
     void* uout;        // output planes (TOut)
+    void* dout;        // FP32-residual stage 2: the increment U2 - U^n (float planes), or null
+    const void* din;   // FP32-residual stage 3: that increment (float planes), or null
     size_t plane;      // cells per plane
     StepCtl* ctl;
     int ty;            // output rows per tile
@@ -418,6 +421,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
 #pragma unroll
         for (int v = 0; v < NV; v++) res[v] = res0[v];
         if constexpr (VIS) add_sources(a.v, u, vol, res);
+        // FP32-residual SSP stages with the state kept in double: stage 2 also writes the
+        // increment D = U2 - U^n = dt/4 (L(U^n) + L(U1)), and stage 3 forms
+        // U^{n+1} = U^n + 2/3 (D + dt L(U2)) from it, so U^n is never rounded to float
+        constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
+        const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             if constexpr (MODE == MODE_RN) {
@@ -425,9 +433,21 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
             } else if constexpr (MODE == MODE_F1) {
                 if (out_lane) store(v, off, TOut(double(res[v]) * ivol));
             } else {
-                double q = double(u[v]) + dt * (double(res[v]) * ivol);
-                if constexpr (MT::use_un) q = MT::ca * un[v] + MT::cb * q;
-                if (out_lane) store(v, off, TOut(q));
+                if (inc && MODE == MODE_F2) {
+                    const double d = 0.25 * dt * (double(ld_plane(lin, P, v, off)) + double(res[v]) * ivol);
+                    if (out_lane) {
+                        st_plane_stream((TOut*)a.dout, P, v, off, TOut(d));
+                        store(v, off, TOut(un[v] + d));
+                    }
+                } else if (inc && MODE == MODE_S3) {
+                    const double d = double(ld_plane((const TIn*)a.din, P, v, off));
+                    const double q = un[v] + (2.0 / 3.0) * (d + dt * (double(res[v]) * ivol));
+                    if (out_lane) store(v, off, TOut(q));
+                } else {
+                    double q = double(u[v]) + dt * (double(res[v]) * ivol);
+                    if constexpr (MT::use_un) q = MT::ca * un[v] + MT::cb * q;
+                    if (out_lane) store(v, off, TOut(q));
+                }
             }
         }
     };

@@ -84,6 +84,9 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     const TOut* __restrict__ lin = (const TOut*)a.lin;
     TOut* uo = (TOut*)a.uout;
     const size_t P = a.plane;
+    // FP32-residual stages 2 and 3 with the increment form (see stage_tile)
+    constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
+    const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
 
     // stencil input at (xg, row r, plane kk); ghosts take their boundary image
     auto load_cell = [&](int r, int kk, R (&u)[NV]) {
@@ -306,6 +309,18 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                             if (out_lane) rn[v] += hx * g.h[1][rf + 1] * hz * __builtin_fabs(double(res));
                         } else if constexpr (MODE == MODE_F1) {
                             if (out_lane) st_plane(uo, P, v, off, TOut(double(res) * ivol));
+                        } else if (inc && MODE == MODE_F2) {
+                            // FP32-residual stage 2: the increment dt/4 (L(U^n) + L(U1)) in float
+                            // beside U2 (stage 3 accumulates U^n + 2/3 (D + dt L(U2)) in double)
+                            const double d = 0.25 * dt * (double(ld_plane(lin, P, v, off)) + double(res) * ivol);
+                            if (out_lane) {
+                                st_plane((TOut*)a.dout, P, v, off, TOut(d));
+                                st_plane(uo, P, v, off, TOut(Nc[v] + d));
+                            }
+                        } else if (inc && MODE == MODE_S3) {
+                            const double d = double(ld_plane((const TIn*)a.din, P, v, off));
+                            const double q = Nc[v] + (2.0 / 3.0) * (d + dt * (double(res) * ivol));
+                            if (out_lane) st_plane(uo, P, v, off, TOut(q));
                         } else {
                             double q = double(Um[v]) + dt * (double(res) * ivol);
                             if constexpr (MT::use_un) q = MT::ca * Nc[v] + MT::cb * q;

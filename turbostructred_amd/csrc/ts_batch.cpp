@@ -192,7 +192,7 @@ size_t state_cells(const ts_config& c) { return (size_t)c.nx * c.ny * c.nz; }
 
 // geometry + state + control of one problem laid out in its arena block
 struct Layout {
-    size_t u0, l, u2, geo, ctl, markers, total;
+    size_t u0, l, u2, d2, geo, ctl, markers, total;  // d2: FP32-residual stage-2 increment
     size_t geo_doubles;
 };
 
@@ -212,6 +212,7 @@ Layout layout_of(const ts_config& c) {
     L.u0 = o; o = align_up(o + n * 5 * 8);
     L.l = o; o = align_up(o + n * 5 * se);
     L.u2 = o; o = align_up(o + n * 5 * se);
+    L.d2 = o; o = align_up(o + (se == 4 ? n * 5 * se : 0));
     L.geo = o; o = align_up(o + geo * 8);
     L.ctl = o; o = align_up(o + sizeof(StepCtl));
     L.markers = o;
@@ -349,9 +350,12 @@ KArgs stage_desc(const Problem& p, int kernel, char* arena) {
     void* u0 = base + L.u0;
     void* l = base + L.l;
     void* u2 = base + L.u2;
+    // FP32-residual: stage 2 writes the increment U2 - U^n beside U2, stage 3 accumulates in
+    // double from it
+    void* d2 = p.cfg.precision == TS_FP32_RESIDUAL ? (void*)(base + L.d2) : nullptr;
     if (kernel == 0) { a.uin = u0; a.uout = l; }
-    else if (kernel == 1) { a.uin = u0; a.lin = l; a.uout = u2; }
-    else { a.uin = u2; a.uout = u0; }
+    else if (kernel == 1) { a.uin = u0; a.lin = l; a.uout = u2; a.dout = d2; }
+    else { a.uin = u2; a.uout = u0; a.din = d2; }
     return a;
 }
 

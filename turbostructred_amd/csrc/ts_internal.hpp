@@ -18,9 +18,9 @@ struct ts_ctx {
     hipStream_t stream = nullptr;
 
     // device state: dU[0] is U^n (double); dU[1], dU[2] are stage arrays (double, or float in
-    // the FP32-residual SSP form).  Forward Euler ping-pongs between dU[0] and dU[1]; cur is the
-    // buffer holding the current state.
-    void* dU[3] = {nullptr, nullptr, nullptr};
+    // the FP32-residual SSP form, which also keeps the stage-2 increment U2 - U^n in dU[3]).
+    // Forward Euler ping-pongs between dU[0] and dU[1]; cur is the buffer holding the current state.
+    void* dU[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t stage_elem = 8;
     int cur = 0;
     // SSP-RK3 with the dt pass fused into stage 1 (dU[1] then holds L(U^n)); off when the

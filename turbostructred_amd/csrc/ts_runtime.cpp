@@ -369,6 +369,10 @@ int ts_algorithmic_bytes_per_cell(const ts_ctx* c, int kind, double* bytes) {
         b[1] = d + s;        // read U^n, write L(U^n)
         b[2] = d + s + s;    // read U^n and L(U^n), write U^2
         b[3] = s + d + d;    // read U^2, U^n, write U^{n+1}
+        if (c->stage_elem == 4) {
+            b[2] += s;  // FP32-residual: also write the increment U^2 - U^n
+            b[3] += s;  // and read it back for the double accumulation of U^{n+1}
+        }
     } else {
         b[1] = d + s;        // read U^n, write U^1
         b[2] = s + d + s;    // read U^1, U^n, write U^2

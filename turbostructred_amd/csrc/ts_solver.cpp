@@ -180,7 +180,8 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     if ((r = dalloc(c, &u0, c->ncell * 5))) return r;
     c->dU[0] = u0;
     c->stage_elem = (k.rk_stages == 3 && k.precision == TS_FP32_RESIDUAL) ? 4 : 8;
-    int nstage = k.rk_stages == 3 ? 2 : 1;
+    // SSP-RK3: L(U^n) / U1 and U2; the FP32-residual form adds the stage-2 increment
+    int nstage = k.rk_stages == 3 ? (c->stage_elem == 4 ? 3 : 2) : 1;
     for (int s = 1; s <= nstage; s++) {
         double* p = nullptr;
         size_t count = c->stage_elem == 8 ? c->ncell * 5 : (c->ncell * 5 + 1) / 2;
@@ -372,11 +373,15 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[MAX_LAUNCH
         norms(c->dU[0]);
         a.lin = c->dU[1];
         a.uout = c->dU[2];
+        a.dout = c->dU[3];  // FP32-residual form: the increment U2 - U^n (null in FP64)
         L[n++] = {c->ks.f2, a, 2};  // U2 from U^n + dt L(U^n) on the fly
         a.lin = nullptr;
+        a.dout = nullptr;
+        a.din = c->dU[3];
         a.uin = c->dU[2];
         a.uout = c->dU[0];
         L[n++] = {c->ks.s3, a, 3};
+        a.din = nullptr;
     } else {
         a.uin = c->dU[0];
         L[n++] = {c->ks.dt, a, 0};

@@ -163,6 +163,22 @@ int ts_set_time(ts_ctx* ctx, double time, long long iteration);
 int ts_step(ts_ctx* ctx, int max_steps, double t_max, double next_snapshot_time,
             ts_step_info* out);
 
+/* Allocation-free step (no counterpart in the reference; for callers that cannot allocate device
+   memory: code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The steps of ts_step on a context created from cfg, starting from (time, iteration), applied
+   in place to d_state: device memory holding the interior state as 5 planes of nx*ny*nz doubles
+   (rho, rho u, rho v, rho w, E; x fastest).  The grid tables, the control block and the stage
+   arrays live in one fixed device region of TS_NOALLOC_SCRATCH_BYTES that the library holds on
+   each GPU it runs on until the process ends; a problem that needs more returns TS_ERR_INVALID
+   before any launch.  Calls no hipMalloc/hipFree/hipHostMalloc and builds no graph.  `stream`
+   is a hipStream_t (NULL: the default stream); the call returns after its work has finished.
+   Every call on one GPU shares that GPU's region: calls that overlap on one GPU race.  No dt
+   log, residual norms or kernel timing; errors are read with ts_last_error(NULL). */
+#define TS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ts_step_noalloc(const ts_config* cfg, double* d_state, double time, long long iteration,
+                    int max_steps, double t_max, double next_snapshot_time, ts_step_info* out,
+                    void* stream);
+
 /* Enable the residual norms sum(V*|R|) per variable of UpdateSolution (kernel.h:326), reported
    in ts_step_info.residual_norm for the last step taken (R of U^n: the forward-Euler residual,
    or SSP-RK3 stage 1).  Off by default: they cost one extra residual pass per ts_step call

@@ -217,6 +217,8 @@ def load(path=LIB_PATH):
     lib.ts_batch_start_server.argtypes = [C.c_void_p]
     lib.ts_batch_stop_server.argtypes = [C.c_void_p]
     lib.ts_autotune.argtypes = [P(Config), P(C.c_int), P(C.c_double)]
+    lib.ts_step_noalloc.argtypes = [P(Config), C.c_void_p, C.c_double, C.c_longlong, C.c_int, C.c_double,
+                                    C.c_double, P(StepInfo), C.c_void_p]
     _lib = lib
     return lib
 
@@ -227,7 +229,7 @@ EXPORTED = [
     "ts_enable_kernel_timing", "ts_get_kernel_timing", "ts_memory_footprint",
     "ts_algorithmic_bytes_per_cell", "ts_batch_create", "ts_batch_destroy",
     "ts_batch_last_error", "ts_batch_submit", "ts_batch_run", "ts_batch_poll",
-    "ts_batch_start_server", "ts_batch_stop_server", "ts_autotune",
+    "ts_batch_start_server", "ts_batch_stop_server", "ts_autotune", "ts_step_noalloc",
 ]
 
 

@@ -74,6 +74,13 @@ struct ts_ctx {
 
     std::vector<double> host_stage;  // SoA staging for uploads / downloads
     size_t device_bytes = 0;
+
+    // Borrowed-memory mode (ts_step_noalloc): device buffers are carved out of `arena` (a fixed
+    // region the library owns for the process), the state buffer dU[0] and the stream are the
+    // caller's, and the control block's host mirror is hctl_local.  Nothing is allocated or freed.
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    tsd::StepCtl hctl_local{};
 };
 
 namespace tsr {
@@ -92,6 +99,8 @@ bool lookup_kernels(int dim, int flux, int rv, int precision, tsd::KernelSet* ks
 void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, double* o);
 // output rows per tile; `slots` = tiles the GPU holds at once (0: unknown)
 int tuned_rows_per_tile(const ts_config& c, long long slots = 0);
+// the calling thread's error text for entry points without a context (ts_last_error(NULL))
+void set_thread_error(const std::string& m);
 }  // namespace tsr
 
 size_t ts_state_size_cfg(const ts_config* c);

@@ -194,6 +194,8 @@ static int ctx_fail(ts_ctx* c, int code, const std::string& m) {
 // ts_last_error(NULL) returns it.
 static thread_local std::string t_create_err;
 
+void tsr::set_thread_error(const std::string& m) { t_create_err = m; }
+
 extern "C" {
 
 int ts_create(const ts_config* cfg, ts_ctx** out) {

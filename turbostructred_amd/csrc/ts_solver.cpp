@@ -23,6 +23,19 @@ namespace {
 
 template <typename T>
 int dalloc(ts_ctx* c, T** p, size_t count) {
+    if (c->arena) {
+        // borrowed-memory mode: bump allocation in the library's fixed region, 256-byte aligned
+        const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (c->arena_used + bytes > c->arena_cap) {
+            c->err = "the problem needs more than the " + std::to_string(c->arena_cap) +
+                     " bytes of the allocation-free scratch region";
+            return TS_ERR_INVALID;
+        }
+        *p = (T*)(c->arena + c->arena_used);
+        c->arena_used += bytes;
+        c->device_bytes += bytes;
+        return TS_OK;
+    }
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, count * sizeof(T));
     if (e != hipSuccess) {
@@ -63,7 +76,7 @@ long long tile_slots(const ts_ctx* c, int tiles_per_block) {
 
 int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     CHK(hipSetDevice(cfg->device));
-    CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (!c->arena) CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const ts_config& k = c->cfg;
     GridArgs& g = c->grid;
     g.nd = c->nd;
@@ -176,9 +189,11 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
     }
     // state buffers
     int r;
-    double* u0 = nullptr;
-    if ((r = dalloc(c, &u0, c->ncell * 5))) return r;
-    c->dU[0] = u0;
+    if (!c->dU[0]) {  // (borrowed-memory mode: the caller's buffer)
+        double* u0 = nullptr;
+        if ((r = dalloc(c, &u0, c->ncell * 5))) return r;
+        c->dU[0] = u0;
+    }
     c->stage_elem = (k.rk_stages == 3 && k.precision == TS_FP32_RESIDUAL) ? 4 : 8;
     // SSP-RK3: L(U^n) / U1 and U2; the FP32-residual form adds the stage-2 increment
     int nstage = k.rk_stages == 3 ? (c->stage_elem == 4 ? 3 : 2) : 1;
@@ -189,7 +204,12 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->dU[s] = p;
     }
     if ((r = dalloc(c, &c->dctl, 1))) return r;
-    CHK(hipHostMalloc((void**)&c->hctl, sizeof(StepCtl), hipHostMallocDefault));
+    if (c->arena) {
+        c->hctl = &c->hctl_local;
+        c->use_graphs = false;  // instantiating a graph allocates
+    } else {
+        CHK(hipHostMalloc((void**)&c->hctl, sizeof(StepCtl), hipHostMallocDefault));
+    }
     std::memset(c->hctl, 0, sizeof(StepCtl));
     // tiling
     if (c->nd == 1) {
@@ -215,7 +235,7 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->block_dim = dim3(64 * ZW);
         c->grid_dim = dim3(c->ntiles);
     }
-    c->host_stage.resize(c->ncell * 5);
+    if (!c->arena) c->host_stage.resize(c->ncell * 5);
     const char* sep = getenv("TS_SEPARATE_DT");
     c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
     c->euler1d_wg = k.rk_stages == 1 && c->nd == 1 && k.precision == TS_FP64 && !c->vis && c->ks.euler1d_chunk &&
@@ -233,9 +253,9 @@ void destroy_device_ctx(ts_ctx* c) {
     if (c->rn_part) (void)hipFree(c->rn_part);
     if (c->d_cells) (void)hipFree(c->d_cells);
     if (c->d_cell_vals) (void)hipFree(c->d_cell_vals);
-    if (c->hctl) (void)hipHostFree(c->hctl);
+    if (c->hctl && c->hctl != &c->hctl_local) (void)hipHostFree(c->hctl);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && !c->arena) (void)hipStreamDestroy(c->stream);
 }
 
 void aos_to_soa(const ts_ctx* c, const double* aos, double* soa) {

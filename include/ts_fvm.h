@@ -133,7 +133,8 @@ typedef struct ts_ctx ts_ctx;
 int ts_create(const ts_config* cfg, ts_ctx** out);
 int ts_destroy(ts_ctx* ctx);
 /* Message of the context's last failure.  ts_last_error(NULL) returns the message of the calling
-   thread's last failed ts_create (no context exists then to hold it). */
+   thread's last failed call that has no context to hold it (ts_create, ts_step_noalloc,
+   ts_eos_eval). */
 const char* ts_last_error(const ts_ctx* ctx);
 
 /* Number of doubles of the reference state array (nXAll*nYAll*nZAll*5, kernel.h:240). */
@@ -202,6 +203,15 @@ int ts_get_kernel_timing(ts_ctx* ctx, int kind, double* total_ms, long long* lau
    per cell; with viscous fluxes, 3*ndims velocity gradients and 4 values per face). */
 int ts_memory_footprint(const ts_ctx* ctx, size_t* device_bytes);
 int ts_algorithmic_bytes_per_cell(const ts_ctx* ctx, int kind, double* bytes);
+
+/* The equation of state on the device: the GeneralEOS interface (generalEOS.h:6-23) as
+   IdealGasEOS implements it (idealGasEOS.h:7-46), the functor the flux kernels evaluate
+   pressures and sound speeds through.  out[i] = method(ro[i], x[i]) for n points of host arrays:
+   method 0 GetPressure(ro, e), 1 GetSoundSpeed(ro, e), 2 GetInternalEnergy(ro, p),
+   3 GetPressureEnergyDerivative(ro, e), 4 GetPressureDensityDerivative(ro, e), with e the
+   specific internal energy.  Errors are read with ts_last_error(NULL). */
+int ts_eos_eval(double gamma, int method, const double* ro, const double* x, size_t n, double* out,
+                int device);
 
 /* ---- batching and serving (extension: many independent problems on one GPU) ------------- */
 typedef struct ts_batch ts_batch;

@@ -219,6 +219,7 @@ def load(path=LIB_PATH):
     lib.ts_autotune.argtypes = [P(Config), P(C.c_int), P(C.c_double)]
     lib.ts_step_noalloc.argtypes = [P(Config), C.c_void_p, C.c_double, C.c_longlong, C.c_int, C.c_double,
                                     C.c_double, P(StepInfo), C.c_void_p]
+    lib.ts_eos_eval.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_size_t, P(C.c_double), C.c_int]
     _lib = lib
     return lib
 
@@ -230,6 +231,7 @@ EXPORTED = [
     "ts_algorithmic_bytes_per_cell", "ts_batch_create", "ts_batch_destroy",
     "ts_batch_last_error", "ts_batch_submit", "ts_batch_run", "ts_batch_poll",
     "ts_batch_start_server", "ts_batch_stop_server", "ts_autotune", "ts_step_noalloc",
+    "ts_eos_eval",
 ]
 
 
@@ -336,6 +338,23 @@ def autotune(cfg):
     if rc != TS_OK:
         raise TSError(rc, "ts_autotune failed")
     return ty.value, ms.value
+
+
+EOS_PRESSURE, EOS_SOUND_SPEED, EOS_INTERNAL_ENERGY, EOS_DP_DE, EOS_DP_DRHO = range(5)
+
+
+def eos_eval(gamma, method, ro, x, device=0):
+    """The device ideal-gas EOS functor (ts_eos_eval) over arrays: method EOS_* of (ro, x)."""
+    lib = load()
+    ro = np.ascontiguousarray(ro, dtype=np.float64).ravel()
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    if ro.size != x.size:
+        raise ValueError("ro and x differ in length")
+    out = np.empty_like(ro)
+    rc = lib.ts_eos_eval(gamma, method, _dp(ro), _dp(x), ro.size, _dp(out), device)
+    if rc != TS_OK:
+        raise TSError(rc, lib.ts_last_error(None).decode())
+    return out
 
 
 class Batch:

@@ -26,43 +26,6 @@ struct BCDev {
     double rho;
 };
 
-// Image state of a boundary condition across a face normal to `axis` (getDummyValues,
-// BCGeneral.h:186-268).  Every condition in the reference is even in the normal's sign, so the
-// ghost fill (normal -e_axis on both sides, kernel.h:743,760) and the boundary-face states
-// (normal +e_axis, ExplicitRungeKuttaFVM.h:1202,1208) share this one function.
-template <typename R>
-__device__ __forceinline__ void bc_image(const BCDev& bc, R gamma, int axis, const R (&u)[NV], R (&o)[NV]) {
-    switch (bc.type) {
-    case BC_SYMMETRY:
-        // U - 2 (n.m) n: only the normal momentum changes sign
-        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
-        o[1 + axis] = -u[1 + axis];
-        break;
-    case BC_WALL:
-        o[0] = u[0]; o[1] = -u[1]; o[2] = -u[2]; o[3] = -u[3]; o[4] = u[4];
-        break;
-    case BC_MOVING_WALL: {
-        R rho = u[0];
-        R ir = R(1) / rho;
-        R vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
-        R rhoe = u[4] - R(0.5) * rho * (vx * vx + vy * vy + vz * vz);
-        R dx = R(2) * R(bc.vel[0]) - vx, dy = R(2) * R(bc.vel[1]) - vy, dz = R(2) * R(bc.vel[2]) - vz;
-        o[0] = rho; o[1] = rho * dx; o[2] = rho * dy; o[3] = rho * dz;
-        o[4] = rhoe + R(0.5) * rho * (dx * dx + dy * dy + dz * dz);
-        break;
-    }
-    case BC_SUBSONIC_INLET: {
-        R r = R(bc.rho);
-        R vx = R(bc.vel[0]), vy = R(bc.vel[1]), vz = R(bc.vel[2]);
-        o[0] = r; o[1] = r * vx; o[2] = r * vy; o[3] = r * vz;
-        o[4] = R(bc.p) / (gamma - R(1)) + R(0.5) * r * (vx * vx + vy * vy + vz * vz);
-        break;
-    }
-    default:  // Natural, SubsonicOutlet: copy
-        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
-    }
-}
-
 // ---- fast reciprocal / square root ---------------------------------------------------------
 // Hardware estimates refined by Newton steps (double) or used directly (float).  Results are
 // within an ulp or two of the correctly rounded value; the parity tolerance is 1e-12.  Only for
@@ -114,6 +77,77 @@ __device__ inline float vabs(float a) { return __builtin_fabsf(a); }
 template <typename R> __device__ inline R tiny_divisor();
 template <> __device__ inline double tiny_divisor<double>() { return 1e-290; }
 template <> __device__ inline float tiny_divisor<float>() { return 1e-35f; }
+
+// ---- equation of state (generalEOS.h:6-23; ideal gas, idealGasEOS.h:7-46) -------------------
+// The reference's GeneralEOS interface as a device functor: pressure, sound speed and specific
+// internal energy, and the two pressure derivatives, as functions of the density and the
+// specific internal energy e (these five follow the reference's formulas operation for
+// operation, with IEEE division and square root).  The kernels work on conserved variables, so
+// the functor also gives the volumetric forms they evaluate: the pressure of an internal energy
+// density rho e, the sound speed from p and 1/rho, and rho e of a pressure -- each the
+// reference's method with the density factor folded in (for an ideal gas
+// GetPressure(rho, rho e / rho) = (gamma - 1) rho e, with no division).  HLLC, its dt
+// eigenvalues, the boundary images and the exact solver's pressures and face energy go through
+// it.  The Roe flux and the exact solver's wave relations are the reference's perfect-gas
+// formulas (RoeSolverPerfectGasEOS.h, GodunovSolverPerfectGasEOS.h) and use gamma directly.
+__device__ inline double ieee_sqrt(double x) { return __builtin_sqrt(x); }
+__device__ inline float ieee_sqrt(float x) { return __builtin_sqrtf(x); }
+
+template <typename R>
+struct IdealGasEOS {
+    R gamma;
+    __device__ explicit IdealGasEOS(R g) : gamma(g) {}
+    // GeneralEOS (generalEOS.h:9-21) as IdealGasEOS implements it (idealGasEOS.h:22-44)
+    __device__ R GetPressure(R ro, R e) const { return (gamma - R(1)) * ro * e; }
+    __device__ R GetSoundSpeed(R ro, R e) const {
+        const R pressure = (gamma - R(1)) * ro * e;
+        return ieee_sqrt(gamma * pressure / ro);
+    }
+    __device__ R GetInternalEnergy(R ro, R pressure) const { return pressure / ((gamma - R(1)) * ro); }
+    __device__ R GetPressureEnergyDerivative(R ro, R) const { return (gamma - R(1)) * ro; }
+    __device__ R GetPressureDensityDerivative(R, R e) const { return (gamma - R(1)) * e; }
+    // volumetric forms of the kernels (rho_e = rho e; inv_ro = 1/rho from the caller's reciprocal)
+    __device__ R PressureOfEnergyDensity(R rho_e) const { return (gamma - R(1)) * rho_e; }
+    __device__ R SoundSpeedOfPressure(R p, R inv_ro) const { return fsqrt(gamma * p * inv_ro); }
+    __device__ R EnergyDensityOfPressure(R p) const { return p / (gamma - R(1)); }
+};
+
+// Image state of a boundary condition across a face normal to `axis` (getDummyValues,
+// BCGeneral.h:186-268).  Every condition in the reference is even in the normal's sign, so the
+// ghost fill (normal -e_axis on both sides, kernel.h:743,760) and the boundary-face states
+// (normal +e_axis, ExplicitRungeKuttaFVM.h:1202,1208) share this one function.
+template <typename R>
+__device__ __forceinline__ void bc_image(const BCDev& bc, R gamma, int axis, const R (&u)[NV], R (&o)[NV]) {
+    switch (bc.type) {
+    case BC_SYMMETRY:
+        // U - 2 (n.m) n: only the normal momentum changes sign
+        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
+        o[1 + axis] = -u[1 + axis];
+        break;
+    case BC_WALL:
+        o[0] = u[0]; o[1] = -u[1]; o[2] = -u[2]; o[3] = -u[3]; o[4] = u[4];
+        break;
+    case BC_MOVING_WALL: {
+        R rho = u[0];
+        R ir = R(1) / rho;
+        R vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+        R rhoe = u[4] - R(0.5) * rho * (vx * vx + vy * vy + vz * vz);
+        R dx = R(2) * R(bc.vel[0]) - vx, dy = R(2) * R(bc.vel[1]) - vy, dz = R(2) * R(bc.vel[2]) - vz;
+        o[0] = rho; o[1] = rho * dx; o[2] = rho * dy; o[3] = rho * dz;
+        o[4] = rhoe + R(0.5) * rho * (dx * dx + dy * dy + dz * dz);
+        break;
+    }
+    case BC_SUBSONIC_INLET: {
+        R r = R(bc.rho);
+        R vx = R(bc.vel[0]), vy = R(bc.vel[1]), vz = R(bc.vel[2]);
+        o[0] = r; o[1] = r * vx; o[2] = r * vy; o[3] = r * vz;
+        o[4] = IdealGasEOS<R>(gamma).EnergyDensityOfPressure(R(bc.p)) + R(0.5) * r * (vx * vx + vy * vy + vz * vz);
+        break;
+    }
+    default:  // Natural, SubsonicOutlet: copy
+        o[0] = u[0]; o[1] = u[1]; o[2] = u[2]; o[3] = u[3]; o[4] = u[4];
+    }
+}
 
 template <typename R> __device__ inline R tiny_spread();
 template <> __device__ inline double tiny_spread<double>() { return 3.0 * 2.2250738585072014e-308; }
@@ -357,13 +391,13 @@ __device__ inline void roe_flux(R gamma, R eps, const R (&UL)[NV], const R (&UR)
 template <int D, typename R>
 __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], R (&F)[NV],
                                  R& lam) {
-    const R gm1 = gamma - R(1);
+    const IdealGasEOS<R> eos(gamma);
     R il = frcp(UL[0]), ir = frcp(UR[0]);
     R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
     R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
-    R pl = gm1 * internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL);
-    R pr = gm1 * internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR);
-    R al = fsqrt(gamma * pl * il), ar = fsqrt(gamma * pr * ir);
+    R pl = eos.PressureOfEnergyDensity(internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL));
+    R pr = eos.PressureOfEnergyDensity(internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR));
+    R al = eos.SoundSpeedOfPressure(pl, il), ar = eos.SoundSpeedOfPressure(pr, ir);
     R ul = vl[D], ur = vr[D];
     R SL = __builtin_fmin(ul - al, ur - ar);
     R SR = __builtin_fmax(ul + al, ur + ar);
@@ -541,14 +575,15 @@ template <int D, typename R>
 __device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)[NV], R (&F)[NV],
                                     R& lam) {
     const double g = double(gammaR);
+    const IdealGasEOS<double> eos(g);
     double UL[NV], UR[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) { UL[i] = double(ULr[i]); UR[i] = double(URr[i]); }
     double il = 1.0 / UL[0], ir = 1.0 / UR[0];
     double vL[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
     double vR[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
-    double pL = (g - 1.0) * UL[0] * (UL[4] * il - (vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2]) / 2.0);
-    double pR = (g - 1.0) * UR[0] * (UR[4] * ir - (vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]) / 2.0);
+    double pL = eos.GetPressure(UL[0], UL[4] * il - (vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2]) / 2.0);
+    double pR = eos.GetPressure(UR[0], UR[4] * ir - (vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]) / 2.0);
     double uL = vL[D], uR = vR[D];
     StarDev st = star_state_dev(g, UL[0], uL, pL, UR[0], uR, pR);
     double ro, u, p;
@@ -593,10 +628,10 @@ __device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)
     } else {
         // Uface and F(Uface) (GodunovSolverPerfectGasEOS.h:416-427)
         double U0 = ro, U1 = ro * Vf[0], U2 = ro * Vf[1], U3 = ro * Vf[2];
-        double U4 = ro * (p / (ro * (g - 1.0)) + 0.5 * (Vf[0] * Vf[0] + Vf[1] * Vf[1] + Vf[2] * Vf[2]));
+        double U4 = ro * (eos.GetInternalEnergy(ro, p) + 0.5 * (Vf[0] * Vf[0] + Vf[1] * Vf[1] + Vf[2] * Vf[2]));
         double iro = 1.0 / U0;
         double vx = U1 * iro, vy = U2 * iro, vz = U3 * iro;
-        double pp = (g - 1.0) * (U4 - U0 * (vx * vx + vy * vy + vz * vz) / 2.0);
+        double pp = eos.PressureOfEnergyDensity(U4 - U0 * (vx * vx + vy * vy + vz * vz) / 2.0);
         double vv[3] = {vx, vy, vz};
         double vn = vv[D];
         Fd[0] = U0 * vn;
@@ -632,14 +667,14 @@ __device__ inline R face_eigen(R gamma, const R (&UL)[NV], const R (&UR)[NV]) {
         R phi2 = R(0.5) * gm1 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         return __builtin_fabs(v[D]) + fsqrt(gm1 * h - phi2);
     } else if constexpr (FLUX == FX_HLLC) {
-        const R gm1 = gamma - R(1);
+        const IdealGasEOS<R> eos(gamma);
         R il = frcp(UL[0]), ir = frcp(UR[0]);
         R ul = UL[1 + D] * il, ur = UR[1 + D] * ir;
         R vl[3] = {UL[1] * il, UL[2] * il, UL[3] * il};
         R vr[3] = {UR[1] * ir, UR[2] * ir, UR[3] * ir};
-        R pl = gm1 * internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL);
-        R pr = gm1 * internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR);
-        R al = fsqrt(gamma * pl * il), ar = fsqrt(gamma * pr * ir);
+        R pl = eos.PressureOfEnergyDensity(internal_energy(UL[4], R(0.5) * UL[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), UL));
+        R pr = eos.PressureOfEnergyDensity(internal_energy(UR[4], R(0.5) * UR[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), UR));
+        R al = eos.SoundSpeedOfPressure(pl, il), ar = eos.SoundSpeedOfPressure(pr, ir);
         R SL = __builtin_fmin(ul - al, ur - ar);
         R SR = __builtin_fmax(ul + al, ur + ar);
         return __builtin_fmax(__builtin_fabs(SL), __builtin_fabs(SR));

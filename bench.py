@@ -33,6 +33,10 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
                            recon=capi.RECON_LINEAR_LIM, limiter=capi.LIMITER_BJ, rk_stages=3,
                            precision=precision, bcs=[dict(type=capi.BC_NATURAL)])
     U = lax_liu_state(cfg)
+    # tile height timed on this state (the library's autotuner); the context below uses the winner
+    chosen, cands = capi.autotune_state(cfg, U, steps=5)
+    tuning = {"chosen_rows_per_tile": chosen, "occupancy_model_rows_per_tile": cands[0][0] if cands else None,
+              "kernel_ms_per_step": {str(r): round(ms, 4) for r, ms in cands}}
     s = capi.Solver(cfg)
     t0 = time.perf_counter()
     s.set_state(U)
@@ -68,7 +72,7 @@ def gpu_run(n, steps, warmup, precision=capi.FP64):
     t0 = time.perf_counter()
     state = s.get_state()
     t_get = time.perf_counter() - t0
-    return cfg, elapsed, info, kern, state, (t_set, t_get)
+    return cfg, elapsed, info, kern, state, (t_set, t_get), tuning
 
 
 def tgv_state(cfg, seed=2025):
@@ -264,24 +268,41 @@ def batched_tubes(count=4096, sample=48, seed=2024):
     a random sample of the same tubes (cell-updates/s of each), with density L1 error."""
     from turbostructred_amd import workloads
     tubes = workloads.shock_tubes(count, seed=seed, precision=capi.FP32_RESIDUAL)
-    b = capi.Batch(hbm_budget_bytes=4 << 30)
-    # warm-up: kernel loading and graph capture on a few tubes
-    w = [b.submit(cfg, U, 20, te) for cfg, U, te in tubes[:8]]
-    while w:
-        b.run(8)
-        w = [t for t in w if not b.poll(t)[0]]
-    t0 = time.perf_counter()
-    tickets = [b.submit(cfg, U, 10**6, te) for cfg, U, te in tubes]
-    while b.run(32) > 0:
-        pass
-    t_batch = time.perf_counter() - t0
-    results = {}
-    for t in tickets:
-        done, st, info = b.poll(t)
-        assert done
-        results[t] = (st, info)
-    cu_batch = sum(cfg.nx * results[t][1].steps_done for (cfg, _, _), t in zip(tubes, tickets))
-    b.close()
+
+    def whole_run(b):
+        t0 = time.perf_counter()
+        tickets = [b.submit(cfg, U, 10**6, te) for cfg, U, te in tubes]
+        while b.run(32) > 0:
+            pass
+        el = time.perf_counter() - t0
+        res = {}
+        for t in tickets:
+            done, st, info = b.poll(t)
+            assert done
+            res[t] = (st, info)
+        return el, tickets, res
+
+    rates = {}
+    for auto in (True, False):
+        b = capi.Batch(hbm_budget_bytes=4 << 30, autotune_tiles=auto)
+        # warm-up: kernel loading and graph capture on a few tubes
+        w = [b.submit(cfg, U, 20, te) for cfg, U, te in tubes[:8]]
+        while w:
+            b.run(8)
+            w = [t for t in w if not b.poll(t)[0]]
+        # with the autotuner: a first whole run that times the candidate tilings as the load
+        # drains, then the measured run with the tilings it chose
+        runs = 2 if auto else 1
+        for r in range(runs):
+            t_batch, tickets, results = whole_run(b)
+            cu_batch = sum(cfg.nx * results[t][1].steps_done for (cfg, _, _), t in zip(tubes, tickets))
+            rates[(auto, r)] = cu_batch / t_batch
+        if auto:
+            trials = b.tuning()
+            keep = (t_batch, tickets, results, cu_batch)
+        b.close()
+    t_batch, tickets, results, cu_batch = keep
+    chosen = {t["load_bucket"]: t["tile_param"] for t in trials if t["chosen"]}
     # one-at-a-time FP64 solves of a sample
     rng = np.random.default_rng(seed + 1)
     idx = rng.choice(count, size=sample, replace=False)
@@ -303,6 +324,9 @@ def batched_tubes(count=4096, sample=48, seed=2024):
     return {"workload": f"{count} shock tubes, N~logU[64,4096] ({sum(c.nx for c, _, _ in tubes)} cells), "
                         "exact Godunov (FP64 Newton) + limited linear, SSP-RK3, FP32 residual, t_end in-domain",
             "cell_updates_per_s": rate_b, "seconds": t_batch,
+            "first_run_while_tuning_cell_updates_per_s": rates[(True, 0)],
+            "load_rule_tiling_cell_updates_per_s": rates[(False, 0)],
+            "tuned_strips_per_work_item_by_log2_cells": {str(k): chosen[k] for k in sorted(chosen)},
             "fp64_one_at_a_time_cell_updates_per_s": rate_s, "fp64_sample_tubes": sample,
             "speedup_vs_fp64_one_at_a_time": rate_b / rate_s,
             "density_l1_vs_fp64_max": max(l1), "density_l1_vs_fp64_mean": float(np.mean(l1))}
@@ -480,7 +504,7 @@ def main():
     # first, while this process has made no HIP call: the child is then not a fork of a process
     # holding the device
     copy_gbs = copy_bandwidth()
-    cfg, elapsed, info, kern, state, (t_set, t_get) = gpu_run(args.n, args.steps, args.warmup, prec)
+    cfg, elapsed, info, kern, state, (t_set, t_get), tuning = gpu_run(args.n, args.steps, args.warmup, prec)
     ncell = args.n * args.n
     value = ncell * args.steps / elapsed
     dom = max(kern.items(), key=lambda kv: kv[1]["avg_us"] * kv[1]["launches"])
@@ -511,6 +535,7 @@ def main():
         "step_roofline_frac": value * bytes_step / HBM_PEAK,
         "algorithmic_bytes_per_cell_step": bytes_step,
         "kernels": kern,
+        "autotune": tuning,
         # the same K steps with the host-buffer hand-over of the boundary included (AoS state with
         # ghosts in via ts_set_state, out via ts_get_state: layout conversion + PCIe copies)
         "pcie_inclusive_cell_updates_per_s": ncell * args.steps / (t_set + elapsed + t_get),
@@ -521,7 +546,7 @@ def main():
     }
     if args.precision == "fp64":
         # the same run with FP32 residual arithmetic (FP64 state): rate and density error
-        _, el32, _, _, st32, _ = gpu_run(args.n, args.steps, args.warmup, capi.FP32_RESIDUAL)
+        _, el32, _, _, st32, _, _ = gpu_run(args.n, args.steps, args.warmup, capi.FP32_RESIDUAL)
         d = np.abs(capi.interior(cfg, st32)[..., 0] - capi.interior(cfg, state)[..., 0])
         out["fp32_residual"] = {"cell_updates_per_s": ncell * args.steps / el32,
                                 "density_l1_vs_fp64": float(d.mean()), "density_linf_vs_fp64": float(d.max())}

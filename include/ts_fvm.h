@@ -221,6 +221,9 @@ typedef struct ts_batch_config {
     size_t hbm_budget_bytes; /* admission limit on the device memory of resident problems */
     int max_active;          /* maximum number of problems advanced together */
     int use_graphs;          /* capture the per-step launches of a fixed batch in a hipGraph */
+    int autotune_tiles;      /* 1: time each candidate tiling per kernel family and load (floor of
+                                log2 of its resident cells) over whole passes and keep the fastest
+                                per cell-step; 0: a load rule.  Results do not depend on it. */
 } ts_batch_config;
 
 int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out);
@@ -250,10 +253,33 @@ int ts_batch_poll(ts_batch* b, long long ticket, int* done, double* out_state,
 int ts_batch_start_server(ts_batch* b);
 int ts_batch_stop_server(ts_batch* b);
 
+/* The batch autotuner's trials so far: one entry per (kernel family, load bucket, candidate
+   tiling) that has been timed.  tile_param is the output rows per tile in 2-D and the 60-cell
+   strips per work item in 1-D; chosen = 1 marks the winner of a bucket whose candidates are all
+   timed.  Writes up to cap entries and their number to *n. */
+typedef struct ts_tile_trial {
+    int ndims, riemann, recon_variant, precision; /* the kernel family */
+    int load_bucket;                              /* floor(log2(resident cells of the family)) */
+    int tile_param;
+    int passes;
+    double ns_per_cell_step;
+    int chosen;
+} ts_tile_trial;
+int ts_batch_tuning(ts_batch* b, ts_tile_trial* out, int cap, int* n);
+
 /* Autotuner: time the candidate tile heights for cfg's kernel family at cfg's size (the
    occupancy-model choice, its neighbours and 4..64 rows) and store the winner in the
    process-wide tuning table used by later ts_create calls.  Results never depend on it. */
 int ts_autotune(const ts_config* cfg, int* chosen_rows_per_tile, double* best_ms);
+
+/* The same autotuner timed on the caller's own state (the reference AoS layout, n_doubles =
+   ts_state_size of cfg's context; NULL = the quiescent state of ts_autotune) over `steps` steps
+   per candidate, so that data-dependent cost (the exact solver's Newton iterations, limiter
+   branches) counts.  Up to `cap` candidates are reported as (rows per tile, kernel ms per step);
+   the winner is stored in the table and returned in *chosen. */
+int ts_autotune_state(const ts_config* cfg, const double* aos_state, size_t n_doubles, int steps,
+                      int* cand_rows_per_tile, double* cand_ms_per_step, int cap, int* n_cand,
+                      int* chosen_rows_per_tile);
 
 #ifdef __cplusplus
 }

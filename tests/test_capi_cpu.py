@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_the_header(tmp_path):
     """offsetof/sizeof of every field, from a C compile of the header, against ctypes"""
     structs = {"ts_config": capi.Config, "ts_bc_config": capi.BCConfig, "ts_step_info": capi.StepInfo,
-               "ts_batch_config": capi.BatchConfig}
+               "ts_batch_config": capi.BatchConfig, "ts_tile_trial": capi.TileTrial}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ts_fvm.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')

@@ -344,3 +344,44 @@ def test_batched_venkat_stretched_equals_single_solve():
     assert info.steps_done == 30 and info.time == rinfo.time
     np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, ref))
     b.close()
+
+
+def test_batch_autotuner_times_every_candidate_and_keeps_results():
+    """Timed tile trials per (family, load bucket): every candidate gets its passes, one winner
+    per bucket, and the results are bitwise those of the load rule's tiling."""
+    quads = [quad(90 + 7 * s, 70 + 5 * s, 200 + s, precision=capi.FP32_RESIDUAL) for s in range(10)]
+    tubes = [tube(300 + 40 * s, 300 + s) for s in range(8)]
+    res = {}
+    for auto in (True, False):
+        b = capi.Batch(hbm_budget_bytes=1 << 28, autotune_tiles=auto)
+        tickets = [b.submit(cfg, U, 48) for cfg, U in quads + tubes]
+        out = drain(b, list(tickets), chunk=2)
+        res[auto] = [out[t] for t in tickets]
+        if auto:
+            trials = b.tuning()
+            for nd in (1, 2):
+                fam = [t for t in trials if t["ndims"] == nd]
+                buckets = {t["load_bucket"] for t in fam}
+                assert len(buckets) == 1, fam  # membership never changed
+                assert sorted(t["tile_param"] for t in fam) == ([1, 2, 4, 8] if nd == 1 else [4, 8, 16, 32])
+                assert all(t["passes"] >= 2 and t["ns_per_cell_step"] > 0 for t in fam), fam
+                assert sum(t["chosen"] for t in fam) == 1, fam
+                print(nd, [(t["tile_param"], round(t["ns_per_cell_step"], 4), t["chosen"]) for t in fam])
+        else:
+            assert b.tuning() == []
+        b.close()
+    for (cfg, _), (a, ia), (o, io) in zip(quads + tubes, res[True], res[False]):
+        np.testing.assert_array_equal(capi.interior(cfg, a), capi.interior(cfg, o))
+        assert ia.time == io.time and ia.steps_done == io.steps_done
+
+
+def test_autotune_on_a_state_reports_its_candidates():
+    cfg, U = quad(300, 200, 77)
+    before, _ = run_alone(cfg, U, 10)
+    chosen, cands = capi.autotune_state(cfg, U, steps=4)
+    rows = [r for r, _ in cands]
+    assert len(cands) >= 4 and len(set(rows)) == len(rows)
+    assert chosen in rows and all(ms > 0 for _, ms in cands)
+    assert min(cands, key=lambda c: c[1])[0] == chosen
+    after, _ = run_alone(cfg, U, 10)  # a context created now uses the tuned height
+    np.testing.assert_array_equal(before, after)

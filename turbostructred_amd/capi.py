@@ -93,6 +93,15 @@ class BatchConfig(C.Structure):
         ("hbm_budget_bytes", C.c_size_t),
         ("max_active", C.c_int),
         ("use_graphs", C.c_int),
+        ("autotune_tiles", C.c_int),
+    ]
+
+
+class TileTrial(C.Structure):
+    _fields_ = [
+        ("ndims", C.c_int), ("riemann", C.c_int), ("recon_variant", C.c_int), ("precision", C.c_int),
+        ("load_bucket", C.c_int), ("tile_param", C.c_int), ("passes", C.c_int),
+        ("ns_per_cell_step", C.c_double), ("chosen", C.c_int),
     ]
 
 
@@ -219,6 +228,9 @@ def load(path=LIB_PATH):
     lib.ts_autotune.argtypes = [P(Config), P(C.c_int), P(C.c_double)]
     lib.ts_step_noalloc.argtypes = [P(Config), C.c_void_p, C.c_double, C.c_longlong, C.c_int, C.c_double,
                                     C.c_double, P(StepInfo), C.c_void_p]
+    lib.ts_autotune_state.argtypes = [P(Config), P(C.c_double), C.c_size_t, C.c_int, P(C.c_int), P(C.c_double),
+                                      C.c_int, P(C.c_int), P(C.c_int)]
+    lib.ts_batch_tuning.argtypes = [C.c_void_p, P(TileTrial), C.c_int, P(C.c_int)]
     lib.ts_eos_eval.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_size_t, P(C.c_double), C.c_int]
     _lib = lib
     return lib
@@ -231,7 +243,7 @@ EXPORTED = [
     "ts_algorithmic_bytes_per_cell", "ts_batch_create", "ts_batch_destroy",
     "ts_batch_last_error", "ts_batch_submit", "ts_batch_run", "ts_batch_poll",
     "ts_batch_start_server", "ts_batch_stop_server", "ts_autotune", "ts_step_noalloc",
-    "ts_eos_eval",
+    "ts_eos_eval", "ts_autotune_state", "ts_batch_tuning",
 ]
 
 
@@ -357,16 +369,36 @@ def eos_eval(gamma, method, ro, x, device=0):
     return out
 
 
+def autotune_state(cfg, aos=None, steps=5, cap=32):
+    """Time the candidate tile heights on a given state (ts_autotune_state): returns
+    (chosen rows per tile, [(rows per tile, kernel ms per step), ...])."""
+    lib = load()
+    rows = (C.c_int * cap)()
+    ms = (C.c_double * cap)()
+    n, ch = C.c_int(), C.c_int()
+    if aos is None:
+        p, nd = None, 0
+    else:
+        a = np.ascontiguousarray(aos, dtype=np.float64).ravel()
+        p, nd = _dp(a), a.size
+    rc = lib.ts_autotune_state(C.byref(cfg), p, nd, steps, rows, ms, cap, C.byref(n), C.byref(ch))
+    if rc != TS_OK:
+        raise TSError(rc, "ts_autotune_state failed")
+    return ch.value, [(rows[i], ms[i]) for i in range(n.value)]
+
+
 class Batch:
     """Owner of one ts_batch: many independent solves advanced together (continuous batching)."""
 
-    def __init__(self, hbm_budget_bytes=8 << 30, max_active=1 << 20, use_graphs=True, device=0):
+    def __init__(self, hbm_budget_bytes=8 << 30, max_active=1 << 20, use_graphs=True, device=0,
+                 autotune_tiles=True):
         self.lib = load()
         bc = BatchConfig()
         bc.device = device
         bc.hbm_budget_bytes = hbm_budget_bytes
         bc.max_active = max_active
         bc.use_graphs = int(use_graphs)
+        bc.autotune_tiles = int(autotune_tiles)
         h = C.c_void_p()
         rc = self.lib.ts_batch_create(C.byref(bc), C.byref(h))
         if rc != TS_OK:
@@ -421,3 +453,10 @@ class Batch:
 
     def stop_server(self):
         self._check(self.lib.ts_batch_stop_server(self.h))
+
+    def tuning(self, cap=256):
+        """The autotuner's timed trials (ts_batch_tuning): list of dicts."""
+        arr = (TileTrial * cap)()
+        n = C.c_int()
+        self._check(self.lib.ts_batch_tuning(self.h, arr, cap, C.byref(n)))
+        return [{f: getattr(arr[i], f) for f, _ in TileTrial._fields_} for i in range(n.value)]

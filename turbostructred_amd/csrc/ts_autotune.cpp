@@ -67,13 +67,20 @@ int tuned_rows_per_tile(const ts_config& c, long long slots) {
 }
 }  // namespace tsr
 
-extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
-    if (!cfg || cfg->ndims == 1) {
+namespace {
+// Time `steps` steps of cfg from the reference AoS state (NULL: a quiescent uniform state) for
+// each candidate tile height; the winner goes into the table.
+int autotune_impl(const ts_config* cfg, const double* aos, size_t n_doubles, int steps, int* cand_rows,
+                  double* cand_ms, int cap, int* n_cand, int* chosen, double* best_ms) {
+    if (n_cand) *n_cand = 0;
+    if (!cfg || steps < 1 || (aos && n_doubles != ts_state_size_cfg(cfg))) return TS_ERR_INVALID;
+    if (cfg->ndims == 1) {
         if (chosen) *chosen = 1;
         if (best_ms) *best_ms = 0;
-        return cfg ? TS_OK : TS_ERR_INVALID;
+        return TS_OK;
     }
-    // the occupancy model's choice (with no table entry) and its neighbours, then fixed heights
+    // the occupancy model's choice (with no table entry) and its neighbours, the heights at which
+    // the tiles just fit one and two rounds of the resident slots, then fixed heights
     {
         std::lock_guard<std::mutex> lk(g_mu);
         g_table.erase(key_of(*cfg));
@@ -82,23 +89,39 @@ extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
     {
         ts_ctx* c = nullptr;
         if (ts_create(cfg, &c) == TS_OK) {
-            cands = {c->ty, c->ty > 5 ? c->ty - c->ty / 4 : 4, c->ty + c->ty / 4 + 1};
+            const int m = c->ty;
+            cands = {m, m > 5 ? m - m / 4 : 4, m + m / 4 + 1};
+            const long long strips = c->nstrips, tiles_m = c->ntiles;
+            const long long zb = cfg->ndims == 3 ? (cfg->nz + tsd::ZOUT - 1) / tsd::ZOUT : 1;
+            // slots implied by the model's choice: its tile count rounded up to a whole round
+            for (long long rounds = 1; rounds <= 2; rounds++) {
+                const long long target = tiles_m * rounds;
+                for (int ty = 4; ty <= 256; ty++)
+                    if (strips * zb * ((cfg->ny + ty - 1) / ty) <= target) {
+                        cands.push_back(ty);
+                        break;
+                    }
+            }
             ts_destroy(c);
         }
     }
-    for (int ty : {4, 8, 16, 32, 64}) cands.push_back(ty);
+    for (int ty : {4, 8, 16, 24, 32, 48, 64}) cands.push_back(ty);
     const int ymax = cfg->ny;
     std::vector<int> uniq;
     for (int ty : cands)
         if (ty >= 1 && ty <= std::max(ymax, 4) && std::find(uniq.begin(), uniq.end(), ty) == uniq.end()) uniq.push_back(ty);
-    size_t nst = ts_state_size_cfg(cfg);
-    std::vector<double> state(nst, 0.0);
-    // a quiescent uniform state: arithmetic cost does not depend on the data
-    for (size_t i = 0; i < nst / 5; i++) {
-        state[i * 5 + 0] = 1.0;
-        state[i * 5 + 4] = 2.5;
+    const size_t nst = ts_state_size_cfg(cfg);
+    std::vector<double> quiet;
+    if (!aos) {
+        // a quiescent uniform state: the arithmetic of the flux kernels barely depends on the data
+        quiet.assign(nst, 0.0);
+        for (size_t i = 0; i < nst / 5; i++) {
+            quiet[i * 5 + 0] = 1.0;
+            quiet[i * 5 + 4] = 2.5;
+        }
+        aos = quiet.data();
     }
-    int best = -1;
+    int best = -1, k = 0;
     double bms = 1e300;
     for (int ty : uniq) {
         {
@@ -107,20 +130,31 @@ extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
         }
         ts_ctx* c = nullptr;
         if (ts_create(cfg, &c) != TS_OK) continue;
-        ts_set_state(c, state.data(), nst);
         ts_step_info info;
-        ts_step(c, 2, 1e300, 0, &info);  // warm-up
-        ts_enable_kernel_timing(c, 1);
-        ts_step(c, 5, 1e300, 0, &info);
+        int rc = ts_set_state(c, aos, nst);
+        if (rc == TS_OK) rc = ts_step(c, 2, 1e300, 0, &info);  // warm-up
         double ms = 0;
         long long n = 0;
-        ts_get_kernel_timing(c, 4, &ms, &n);
+        if (rc == TS_OK) {
+            ts_enable_kernel_timing(c, 1);
+            rc = ts_step(c, steps, 1e300, 0, &info);
+            ts_get_kernel_timing(c, 4, &ms, &n);
+            if (info.steps_done < steps) rc = TS_ERR_UNPHYSICAL;  // the state ended the run early
+        }
         ts_destroy(c);
-        if (n > 0 && ms < bms) {
+        if (rc != TS_OK || n == 0) continue;
+        ms /= steps;
+        if (cand_rows && cand_ms && k < cap) {
+            cand_rows[k] = ty;
+            cand_ms[k] = ms;
+        }
+        k++;
+        if (ms < bms) {
             bms = ms;
             best = ty;
         }
     }
+    if (n_cand) *n_cand = std::min(k, cap);
     std::lock_guard<std::mutex> lk(g_mu);
     if (best < 0) {
         g_table.erase(key_of(*cfg));
@@ -128,6 +162,17 @@ extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
     }
     g_table[key_of(*cfg)] = best;
     if (chosen) *chosen = best;
-    if (best_ms) *best_ms = bms / 5.0;
+    if (best_ms) *best_ms = bms;
     return TS_OK;
+}
+}  // namespace
+
+extern "C" int ts_autotune(const ts_config* cfg, int* chosen, double* best_ms) {
+    return autotune_impl(cfg, nullptr, 0, 5, nullptr, nullptr, 0, nullptr, chosen, best_ms);
+}
+
+extern "C" int ts_autotune_state(const ts_config* cfg, const double* aos_state, size_t n_doubles, int steps,
+                                 int* cand_rows, double* cand_ms_per_step, int cap, int* n_cand, int* chosen) {
+    if (cap < 0 || (cap > 0 && (!cand_rows || !cand_ms_per_step))) return TS_ERR_INVALID;
+    return autotune_impl(cfg, aos_state, n_doubles, steps, cand_rows, cand_ms_per_step, cap, n_cand, chosen, nullptr);
 }

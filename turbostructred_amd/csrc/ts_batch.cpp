@@ -115,6 +115,26 @@ struct Problem {
     int error = 0;
 };
 
+// Tile-shape autotuning of a family (ts_batch_config.autotune_tiles): for each load bucket
+// (floor(log2) of the family's resident cells) every candidate tiling is timed over whole
+// passes, and the fastest per cell-step is kept.  The candidates take their passes in the
+// palindromic order 0 1 2 3 3 2 1 0, so that a load that drifts steadily within the bucket (solves
+// finishing) weighs on every candidate alike.  Results do not depend on the tiling.
+constexpr int kTileCands = 4;
+constexpr int kTrialPasses = 2;
+constexpr int kTrialOrder[kTileCands * kTrialPasses] = {0, 1, 2, 3, 3, 2, 1, 0};
+constexpr int kCand2D[kTileCands] = {4, 8, 16, 32};  // output rows per tile
+constexpr int kCand1D[kTileCands] = {1, 2, 4, 8};    // 60-cell strips per work item
+
+struct TileTrial {
+    double ms = 0, cell_steps = 0;
+    int passes = 0;
+};
+struct LoadTuning {
+    TileTrial t[kTileCands];
+    int chosen = -1;
+};
+
 struct Family {
     FamilyKey key;
     KernelSet ks;
@@ -131,6 +151,12 @@ struct Family {
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    // autotuning: trials per load bucket, the tiling in use, and the timing of the current pass
+    std::map<int, LoadTuning> tuning;
+    int bucket = -1, cand = -1;  // cand -1: the load rule
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool timed = false;
+    double pass_cell_steps = 0;
 };
 
 }  // namespace
@@ -364,8 +390,27 @@ KArgs stage_desc(const Problem& p, int kernel, char* arena) {
 // of the persistent grid's wavefronts.  Under load, tall tiles spend less on the two warm-up
 // rows and the per-tile dt atomics; a lightly loaded batch keeps short tiles so that each step
 // stays short.
+int next_candidate(const LoadTuning& lt) {
+    if (lt.chosen >= 0) return lt.chosen;
+    int done = 0;
+    for (const TileTrial& t : lt.t) done += t.passes;
+    return kTrialOrder[done < kTileCands * kTrialPasses ? done : kTileCands * kTrialPasses - 1];
+}
+
+// the candidate the autotuner runs next for the family's current load (-1: autotuning off)
+int tuning_candidate(ts_batch* b, Family& f) {
+    long long cells = 0;
+    for (const Problem* q : f.members) cells += (long long)q->ncell;
+    int bucket = 0;
+    while (bucket < 62 && (1ll << (bucket + 1)) <= cells) bucket++;
+    f.bucket = bucket;
+    if (!b->bc.autotune_tiles) return -1;
+    return next_candidate(f.tuning[bucket]);
+}
+
 void retile(ts_batch* b, Family& f) {
     const long long waves = (long long)b->ncu * f.occupancy * 4 / 2;
+    f.cand = tuning_candidate(b, f);
     if (std::get<0>(f.key) == 1) {
         // 1-D: a work item is a group of G consecutive 60-cell strips (G = 8/4/2/1, the largest
         // that still gives half of the wavefronts an item): fewer descriptor loads and dt atomics
@@ -378,6 +423,7 @@ void retile(ts_batch* b, Family& f) {
                 break;
             }
         }
+        if (f.cand >= 0) G = kCand1D[f.cand];
         for (Problem* q : f.members) {
             Problem& p = *q;
             p.desc.ty = G;
@@ -398,6 +444,7 @@ void retile(ts_batch* b, Family& f) {
             break;
         }
     }
+    if (f.cand >= 0) ty = kCand2D[f.cand];
     for (Problem* q : f.members) {
         Problem& p = *q;
         p.desc.ty = ty;
@@ -463,10 +510,17 @@ void issue_step(ts_batch* b, Family& f) {
     f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_ntiles, nblk, b->stream);
 }
 
-int issue_steps(ts_batch* b, Family& f, int n) {
+// `timed`: events bracket the launches themselves (a graph is built before the first event)
+int issue_steps(ts_batch* b, Family& f, int n, bool timed) {
+    if (timed) {
+        for (int i = 0; i < 2; i++)
+            if (!f.ev[i]) BCHK(hipEventCreate(&f.ev[i]));
+    }
     if (!b->bc.use_graphs) {
+        if (timed) BCHK(hipEventRecord(f.ev[0], b->stream));
         for (int s = 0; s < n; s++) issue_step(b, f);
         BCHK(hipGetLastError());
+        if (timed) BCHK(hipEventRecord(f.ev[1], b->stream));
         return TS_OK;
     }
     if (!f.graph || f.graph_steps != n) {
@@ -484,7 +538,34 @@ int issue_steps(ts_batch* b, Family& f, int n) {
         }
         f.graph_steps = n;
     }
+    if (timed) BCHK(hipEventRecord(f.ev[0], b->stream));
     BCHK(hipGraphLaunch(f.graph, b->stream));
+    if (timed) BCHK(hipEventRecord(f.ev[1], b->stream));
+    return TS_OK;
+}
+
+// after the pass has synchronised: credit the timed pass to its candidate, and pick the bucket's
+// winner once every candidate has its passes (a new candidate or the winner needs a retile)
+int credit_trial(ts_batch* b, Family& f) {
+    if (!f.timed) return TS_OK;
+    f.timed = false;
+    float ms = 0;
+    BCHK(hipEventElapsedTime(&ms, f.ev[0], f.ev[1]));
+    LoadTuning& lt = f.tuning[f.bucket];
+    if (lt.chosen >= 0 || f.cand < 0) return TS_OK;
+    TileTrial& t = lt.t[f.cand];
+    t.ms += ms;
+    t.cell_steps += f.pass_cell_steps;
+    t.passes++;
+    bool all = true;
+    for (const TileTrial& x : lt.t) all = all && x.passes >= kTrialPasses;
+    if (all) {
+        int k = 0;
+        for (int c = 1; c < kTileCands; c++)
+            if (lt.t[c].ms * lt.t[k].cell_steps < lt.t[k].ms * lt.t[c].cell_steps) k = c;
+        lt.chosen = k;
+    }
+    if (next_candidate(lt) != f.cand) f.dirty = true;
     return TS_OK;
 }
 
@@ -565,6 +646,8 @@ int ts_batch_destroy(ts_batch* b) {
         if (f.d_desc) (void)hipFree(f.d_desc);
         if (f.d_table) (void)hipFree(f.d_table);
         if (f.d_ntiles) (void)hipFree(f.d_ntiles);
+        for (hipEvent_t e : f.ev)
+            if (e) (void)hipEventDestroy(e);
     }
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     if (b->h_ptr) (void)hipHostFree(b->h_ptr);
@@ -698,8 +781,14 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
                 int r = rebuild_tables(b, f);
                 if (r) return r;
             }
-            int r = issue_steps(b, f, n_steps);
+            const bool timed = f.cand >= 0 && f.tuning[f.bucket].chosen < 0;
+            int r = issue_steps(b, f, n_steps, timed);
             if (r) return r;
+            if (timed) {
+                f.timed = true;
+                f.pass_cell_steps = 0;
+                for (const Problem* q : f.members) f.pass_cell_steps += double(q->ncell) * n_steps;
+            }
         }
         // control blocks of every resident solve back to the host
         if (b->h_ctl_cap < resident.size()) {
@@ -725,6 +814,10 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
         BCHK(hipGetLastError());
         BCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, nres * sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
         BCHK(hipStreamSynchronize(b->stream));
+        for (size_t fi = 0; fi < nfam; fi++) {
+            int r = credit_trial(b, b->families[fi]);
+            if (r) return r;
+        }
         std::vector<int> fin;
         for (int i = 0; i < nres; i++) {
             const StepCtl& c = b->h_ctl[i];
@@ -818,6 +911,40 @@ int ts_batch_stop_server(ts_batch* b) {
     b->cv.notify_all();
     if (b->server.joinable()) b->server.join();
     b->serving = false;
+    return TS_OK;
+}
+
+int ts_batch_tuning(ts_batch* b, ts_tile_trial* out, int cap, int* n) {
+    if (!b || !n || cap < 0 || (cap > 0 && !out)) return TS_ERR_INVALID;
+    std::lock_guard<std::mutex> run(b->run_mu);  // the pass that updates the trials is not running
+    size_t nfam;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        nfam = b->families.size();
+    }
+    int k = 0;
+    for (size_t fi = 0; fi < nfam; fi++) {
+        const Family& f = b->families[fi];
+        for (const auto& kv : f.tuning)
+            for (int c = 0; c < kTileCands; c++) {
+                const TileTrial& t = kv.second.t[c];
+                if (t.passes == 0) continue;
+                if (k < cap) {
+                    ts_tile_trial& o = out[k];
+                    o.ndims = std::get<0>(f.key);
+                    o.riemann = std::get<1>(f.key);
+                    o.recon_variant = std::get<2>(f.key);
+                    o.precision = std::get<3>(f.key);
+                    o.load_bucket = kv.first;
+                    o.tile_param = o.ndims == 1 ? kCand1D[c] : kCand2D[c];
+                    o.passes = t.passes;
+                    o.ns_per_cell_step = t.cell_steps > 0 ? t.ms * 1e6 / t.cell_steps : 0.0;
+                    o.chosen = kv.second.chosen == c ? 1 : 0;
+                }
+                k++;
+            }
+    }
+    *n = k < cap ? k : cap;
     return TS_OK;
 }
 

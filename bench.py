@@ -28,15 +28,17 @@ def lax_liu_state(cfg):
     return util.quadrants(cfg)
 
 
-def gpu_run(n, steps, warmup, precision=capi.FP64):
+def gpu_run(n, steps, warmup, precision=capi.FP64, autotune=True):
     cfg = capi.make_config(ndims=2, nx=n, ny=n, cfl=0.45, riemann=capi.RIEMANN_HLLC,
                            recon=capi.RECON_LINEAR_LIM, limiter=capi.LIMITER_BJ, rk_stages=3,
                            precision=precision, bcs=[dict(type=capi.BC_NATURAL)])
     U = lax_liu_state(cfg)
     # tile height timed on this state (the library's autotuner); the context below uses the winner
-    chosen, cands = capi.autotune_state(cfg, U, steps=5)
-    tuning = {"chosen_rows_per_tile": chosen, "occupancy_model_rows_per_tile": cands[0][0] if cands else None,
-              "kernel_ms_per_step": {str(r): round(ms, 4) for r, ms in cands}}
+    tuning = None
+    if autotune:
+        chosen, cands = capi.autotune_state(cfg, U, steps=5)
+        tuning = {"chosen_rows_per_tile": chosen, "occupancy_model_rows_per_tile": cands[0][0] if cands else None,
+                  "kernel_ms_per_step": {str(r): round(ms, 4) for r, ms in cands}}
     s = capi.Solver(cfg)
     t0 = time.perf_counter()
     s.set_state(U)
@@ -495,6 +497,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-viscous", action="store_true")
     ap.add_argument("--no-sod", action="store_true")
+    # the occupancy model's tiling without the autotuner's trial launches (for profiler runs, whose
+    # per-kernel averages the trials would otherwise enter)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the library has no multi-GPU path); got --gpus {args.gpus}",
@@ -504,7 +509,8 @@ def main():
     # first, while this process has made no HIP call: the child is then not a fork of a process
     # holding the device
     copy_gbs = copy_bandwidth()
-    cfg, elapsed, info, kern, state, (t_set, t_get), tuning = gpu_run(args.n, args.steps, args.warmup, prec)
+    cfg, elapsed, info, kern, state, (t_set, t_get), tuning = gpu_run(args.n, args.steps, args.warmup, prec,
+                                                                      not args.no_autotune)
     ncell = args.n * args.n
     value = ncell * args.steps / elapsed
     dom = max(kern.items(), key=lambda kv: kv[1]["avg_us"] * kv[1]["launches"])
@@ -546,7 +552,8 @@ def main():
     }
     if args.precision == "fp64":
         # the same run with FP32 residual arithmetic (FP64 state): rate and density error
-        _, el32, _, _, st32, _, _ = gpu_run(args.n, args.steps, args.warmup, capi.FP32_RESIDUAL)
+        _, el32, _, _, st32, _, _ = gpu_run(args.n, args.steps, args.warmup, capi.FP32_RESIDUAL,
+                                            not args.no_autotune)
         d = np.abs(capi.interior(cfg, st32)[..., 0] - capi.interior(cfg, state)[..., 0])
         out["fp32_residual"] = {"cell_updates_per_s": ncell * args.steps / el32,
                                 "density_l1_vs_fp64": float(d.mean()), "density_linf_vs_fp64": float(d.max())}

@@ -388,30 +388,53 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         else st_plane_stream(uo, P, v, off, x);
     };
 
-    // one stencil value: the input state at a (possibly ghost) cell of row r
-    auto load_row = [&](int r, R (&u)[NV]) {
+    // One stencil value, the input state at a (possibly ghost) cell of row r, in two steps: the
+    // loads (issue_row), and the value formed from them with its ghost image (complete_row).  The
+    // prefetched row is completed one iteration after its loads are issued, so that no arithmetic
+    // on the loaded values (stage 2's U^n + dt L, a ghost image) makes the wavefront wait for them
+    // early.
+    struct RawRow {
+        TIn u[NV];
+        TOut l[MODE == MODE_F2 ? NV : 1];
+        int rs, yside;
+    };
+    auto issue_row = [&](int r, RawRow& q) {
         int rs = 0, yside = 0;
         if constexpr (DIM > 1) map_index(r, ny, g.periodic[1], rs, yside);
+        q.rs = rs;
+        q.yside = yside;
         const unsigned off = unsigned(rs) * unsigned(nx) + unsigned(xs);
+#pragma unroll
+        for (int v = 0; v < NV; v++) q.u[v] = ld_plane(uin, P, v, off);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++)
-                u[v] = R(double(ld_plane(uin, P, v, off)) + dt * double(ld_plane(lin, P, v, off)));
+            for (int v = 0; v < NV; v++) q.l[v] = ld_plane(lin, P, v, off);
+        }
+    };
+    auto complete_row = [&](const RawRow& q, R (&u)[NV]) {
+        if constexpr (MODE == MODE_F2) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = R(double(q.u[v]) + dt * double(q.l[v]));
         } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(ld_plane(uin, P, v, off));
+            for (int v = 0; v < NV; v++) u[v] = R(q.u[v]);
         }
         if (xside != 0) {
             R t[NV];
-            bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, rs)], gamma, 0, u, t);
+            bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, q.rs)], gamma, 0, u, t);
 #pragma unroll
             for (int v = 0; v < NV; v++) u[v] = t[v];
-        } else if (yside != 0) {
+        } else if (q.yside != 0) {
             R t[NV];
-            bc_image(g.bc[bc_marker(g, yside < 0 ? 2 : 3, xs)], gamma, 1, u, t);
+            bc_image(g.bc[bc_marker(g, q.yside < 0 ? 2 : 3, xs)], gamma, 1, u, t);
 #pragma unroll
             for (int v = 0; v < NV; v++) u[v] = t[v];
         }
+    };
+    auto load_row = [&](int r, R (&u)[NV]) {
+        RawRow q;
+        issue_row(r, q);
+        complete_row(q, u);
     };
 
     // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
@@ -515,7 +538,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         // ---- march along y ----
         const int y0 = yb * a.ty;
         const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
-        R Pn[NV];               // prefetched stencil row (enters the window one iteration later)
+        RawRow Pn;              // prefetched stencil row (completed and entered into the window one
+                                // iteration later)
         double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
         R ytop[NV];             // +y face state of the previously reconstructed row
         // The three-row window (rows rc-1, rc, rc+1) and the values carried to the next row live
@@ -527,7 +551,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         // residual carried to the pending row: its x part plus S_y times the y-flux through its
         // bottom face; the finalize subtracts S_y times the flux through its top face
         __shared__ R carry[4][NV][64];
-        const int wl = (threadIdx.x >> 6) & 3;
+        const int wl = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
         R (&cy)[NV][64] = carry[wl];
         int sm = 0, s0 = 1, sp = 2;  // slots of rows rc-1, rc, rc+1
         // x-neighbours are read from the same LDS rows at lane -+ 1 (the end lanes, halo lanes
@@ -545,15 +569,19 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 ring[wl][s0][v][lane] = t1[v];
             }
         }
-        load_row(y0, Pn);
+        issue_row(y0, Pn);
 #pragma unroll
         for (int v = 0; v < NV; v++) Nc[v] = 0.0;
         const int nit = yend - y0 + 2;
         for (int t = 0; t < nit; t++) {
             const int rc = y0 - 1 + t;  // row being reconstructed
+            {
+                R un1[NV];
+                complete_row(Pn, un1);
 #pragma unroll
-            for (int v = 0; v < NV; v++) ring[wl][sp][v][lane] = Pn[v];
-            if (t + 1 < nit) load_row(rc + 2, Pn);
+                for (int v = 0; v < NV; v++) ring[wl][sp][v][lane] = un1[v];
+            }
+            if (t + 1 < nit) issue_row(rc + 2, Pn);
             if constexpr (MT::use_un) {
                 // U^n of the row finalized below, loaded now: the reconstruction and the y-face
                 // flux cover its latency (it was read three iterations ago as a stencil row, so
@@ -720,7 +748,10 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
         }
     }
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // the wavefront index is the same in all its lanes: readfirstlane lets the compiler keep the
+    // tile, its rows and every row's geometry in scalar registers (scalar loads, whose waits do
+    // not also wait for the vector loads of the stencil prefetch)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * nwaves + wave;
     if constexpr (MODE != MODE_RN) {

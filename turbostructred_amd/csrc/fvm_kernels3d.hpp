@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     }
     const GridArgs& g = a.g;
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar plane indices
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int nyb = (ny + a.ty - 1) / a.ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);  // one tile per workgroup
@@ -88,40 +88,61 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
     const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
 
-    // stencil input at (xg, row r, plane kk); ghosts take their boundary image
-    auto load_cell = [&](int r, int kk, R (&u)[NV]) {
+    // Stencil input at (xg, row r, plane kk), in two steps: the loads (issue_cell), and the
+    // value formed from them with its ghost image (complete_cell).  Prefetched cells are
+    // completed one iteration after their loads are issued, so that no arithmetic on the loaded
+    // values makes the wavefront wait for them early.
+    struct RawCell {
+        TIn u[NV];
+        TOut l[MODE == MODE_F2 ? NV : 1];
         int rs, yside, kks, zside;
-        map_index(r, ny, g.periodic[1], rs, yside);
-        map_index(kk, nz, g.periodic[2], kks, zside);
-        const unsigned off = (unsigned(kks) * unsigned(ny) + unsigned(rs)) * unsigned(nx) + unsigned(xs);
+    };
+    auto issue_cell = [&](int r, int kk, RawCell& q) {
+        map_index(r, ny, g.periodic[1], q.rs, q.yside);
+        map_index(kk, nz, g.periodic[2], q.kks, q.zside);
+        const unsigned off = (unsigned(q.kks) * unsigned(ny) + unsigned(q.rs)) * unsigned(nx) + unsigned(xs);
+#pragma unroll
+        for (int v = 0; v < NV; v++) q.u[v] = ld_plane(uin, P, v, off);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++)
-                u[v] = R(double(ld_plane(uin, P, v, off)) + dt * double(ld_plane(lin, P, v, off)));
+            for (int v = 0; v < NV; v++) q.l[v] = ld_plane(lin, P, v, off);
+        }
+    };
+    auto complete_cell = [&](const RawCell& q, R (&u)[NV]) {
+        if constexpr (MODE == MODE_F2) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) u[v] = R(double(q.u[v]) + dt * double(q.l[v]));
         } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) u[v] = R(ld_plane(uin, P, v, off));
+            for (int v = 0; v < NV; v++) u[v] = R(q.u[v]);
         }
         // one call per axis, so that the axis is a constant inside bc_image (a run-time axis
         // would index the state arrays dynamically and put them in scratch memory)
         R t[NV];
         bool img = true;
-        if (xside != 0) bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, kks * ny + rs)], gamma, 0, u, t);
-        else if (yside != 0) bc_image(g.bc[bc_marker(g, yside < 0 ? 2 : 3, kks * nx + xs)], gamma, 1, u, t);
-        else if (zside != 0) bc_image(g.bc[bc_marker(g, zside < 0 ? 4 : 5, rs * nx + xs)], gamma, 2, u, t);
+        if (xside != 0) bc_image(g.bc[bc_marker(g, xside < 0 ? 0 : 1, q.kks * ny + q.rs)], gamma, 0, u, t);
+        else if (q.yside != 0) bc_image(g.bc[bc_marker(g, q.yside < 0 ? 2 : 3, q.kks * nx + xs)], gamma, 1, u, t);
+        else if (q.zside != 0) bc_image(g.bc[bc_marker(g, q.zside < 0 ? 4 : 5, q.rs * nx + xs)], gamma, 2, u, t);
         else img = false;
         if (img) {
 #pragma unroll
             for (int v = 0; v < NV; v++) u[v] = t[v];
         }
     };
+    auto load_cell = [&](int r, int kk, R (&u)[NV]) {
+        RawCell q;
+        issue_cell(r, kk, q);
+        complete_cell(q, u);
+    };
 
     double block_min = __builtin_inf();
     double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int y0 = yb * a.ty;
     const int yend = (y0 + a.ty < ny) ? y0 + a.ty : ny;
-    R Um[NV], U0[NV], Up[NV], Pn[NV];
+    R Um[NV], U0[NV], Up[NV];
+    RawCell Pn;             // prefetched stencil row, completed one iteration later
     R Zx[NV];               // centre row of the z-neighbour no wavefront holds (edge wavefronts)
+    RawCell Zq;             // its prefetch
     double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
     R ytop[NV];             // +y face state of the previous row
     // carried to the next row in LDS, per wavefront and lane (no barrier needed): the y-flux
@@ -136,11 +157,12 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     R lam_gb = R(0);
     load_cell(y0 - 2, k, Um);
     load_cell(y0 - 1, k, U0);
-    load_cell(y0, k, Pn);
+    issue_cell(y0, k, Pn);
 #pragma unroll
     for (int v = 0; v < NV; v++) { Nc[v] = 0.0; dfx[v][lane] = dhz[v][lane] = gbot[v][lane] = R(0); }
     const bool edge = (w == 0) || (w == ZW - 1);
     const int zoff = (w == 0) ? -1 : 1;  // the z-neighbour an edge wavefront loads itself
+    if (edge) issue_cell(y0 - 1, k + zoff, Zq);
     // the z-flux below this plane for the previous row is still in lds_h[w] when it is read
     R lamzc = R(0), Szc = R(0);
     R Szb = R(0);  // z-face area of the pending row (its y size differs from the current row's)
@@ -149,10 +171,12 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     for (int t = 0; t < nit; t++) {
         const int rc = y0 - 1 + t;  // row being reconstructed
         const bool row_out = rc >= y0 && rc < yend;
-#pragma unroll
-        for (int v = 0; v < NV; v++) Up[v] = Pn[v];
-        if (t + 1 < nit) load_cell(rc + 2, k, Pn);
-        if (edge) load_cell(rc, k + zoff, Zx);
+        complete_cell(Pn, Up);
+        if (t + 1 < nit) issue_cell(rc + 2, k, Pn);
+        if (edge) {
+            complete_cell(Zq, Zx);
+            if (t + 1 < nit) issue_cell(rc + 1, k + zoff, Zq);
+        }
         if constexpr (MT::use_un) {
             // U^n of the row finalized in this iteration (read earlier as a stencil row)
             if (out_plane && t >= 2) {

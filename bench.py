@@ -500,6 +500,7 @@ def main():
     # the occupancy model's tiling without the autotuner's trial launches (for profiler runs, whose
     # per-kernel averages the trials would otherwise enter)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tgv-steps", type=int, default=100)  # timed steps of the 3-D leg
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the library has no multi-GPU path); got --gpus {args.gpus}",
@@ -558,7 +559,7 @@ def main():
         out["fp32_residual"] = {"cell_updates_per_s": ncell * args.steps / el32,
                                 "density_l1_vs_fp64": float(d.mean()), "density_linf_vs_fp64": float(d.max())}
     if not args.no_3d:
-        out["tgv_3d"] = tgv_3d()
+        out["tgv_3d"] = tgv_3d(steps=args.tgv_steps)
     if not args.no_viscous:
         out["viscous_channel"] = viscous_channel()
     if not args.no_sod:

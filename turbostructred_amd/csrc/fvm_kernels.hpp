@@ -29,6 +29,8 @@
 //            ts_step call (the norms reported), so a call costs one extra residual pass.
 #pragma once
 
+#include <type_traits>
+
 #include "fvm_device.hpp"
 
 namespace tsd {
@@ -437,18 +439,28 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         complete_row(q, u);
     };
 
+    // FP32-residual SSP stages with the state kept in double: stage 2 also writes the increment
+    // D = U2 - U^n = dt/4 (L(U^n) + L(U1)), and stage 3 forms U^{n+1} = U^n + 2/3 (D + dt L(U2))
+    // from it, so U^n is never rounded to float.  The centre value of L(U^n) (stage 2) or of D
+    // (stage 3) is loaded with the centre U^n, ahead of the finalize that uses it.
+    constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
+    const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
+    using IncT = typename std::conditional<MODE == MODE_F2, TOut, TIn>::type;
+    const IncT* incsrc = (MODE == MODE_F2) ? (const IncT*)a.lin : (const IncT*)a.din;
+    auto load_inc = [&](unsigned off, IncT (&d)[NV]) {
+        if (inc) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) d[v] = ld_plane(incsrc, P, v, off);
+        }
+    };
+
     // write the result for a finalized cell: the stage update, or L = R/V (MODE_F1)
     auto finish = [&](unsigned off, const R (&u)[NV], const R (&res0)[NV], const double (&un)[NV],
-                      double ivol, double vol) {
+                      const IncT (&dc)[NV], double ivol, double vol) {
         R res[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) res[v] = res0[v];
         if constexpr (VIS) add_sources(a.v, u, vol, res);
-        // FP32-residual SSP stages with the state kept in double: stage 2 also writes the
-        // increment D = U2 - U^n = dt/4 (L(U^n) + L(U1)), and stage 3 forms
-        // U^{n+1} = U^n + 2/3 (D + dt L(U2)) from it, so U^n is never rounded to float
-        constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
-        const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             if constexpr (MODE == MODE_RN) {
@@ -457,13 +469,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 if (out_lane) store(v, off, TOut(double(res[v]) * ivol));
             } else {
                 if (inc && MODE == MODE_F2) {
-                    const double d = 0.25 * dt * (double(ld_plane(lin, P, v, off)) + double(res[v]) * ivol);
+                    const double d = 0.25 * dt * (double(dc[v]) + double(res[v]) * ivol);
                     if (out_lane) {
                         st_plane_stream((TOut*)a.dout, P, v, off, TOut(d));
                         store(v, off, TOut(un[v] + d));
                     }
                 } else if (inc && MODE == MODE_S3) {
-                    const double d = double(ld_plane((const TIn*)a.din, P, v, off));
+                    const double d = double(dc[v]);
                     const double q = un[v] + (2.0 / 3.0) * (d + dt * (double(res[v]) * ivol));
                     if (out_lane) store(v, off, TOut(q));
                 } else {
@@ -483,6 +495,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         double un[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) un[v] = MT::use_un ? ld_plane(a.un, P, v, unsigned(xs)) : 0.0;
+        IncT dc[NV];
+        load_inc(unsigned(xs), dc);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             nb[0][v] = dpp_from_left(c[v]);
@@ -532,7 +546,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 R Fr = dpp_from_right(F[v]);
                 res[v] = -((Fr - F[v]) * Sx);
             }
-            finish(unsigned(xs), c, res, un, ihx, hx);
+            finish(unsigned(xs), c, res, un, dc, ihx, hx);
         }
     } else {
         // ---- march along y ----
@@ -541,6 +555,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         RawRow Pn;              // prefetched stencil row (completed and entered into the window one
                                 // iteration later)
         double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
+        IncT Dc[NV];            // its L(U^n) or increment (FP32-residual stages 2, 3)
         R ytop[NV];             // +y face state of the previously reconstructed row
         // The three-row window (rows rc-1, rc, rc+1) and the values carried to the next row live
         // in LDS, one set per wavefront; no other lane reads them, so no barrier is needed.  The
@@ -590,6 +605,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                     const unsigned offn = unsigned(rc - 1) * unsigned(nx) + unsigned(xs);
 #pragma unroll
                     for (int v = 0; v < NV; v++) Nc[v] = ld_plane(a.un, P, v, offn);
+                    load_inc(offn, Dc);
                 }
             }
             int rcs, rcside;
@@ -659,7 +675,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                             res[v] = cy[v][lane] - G[v] * Sy;
                             Um[v] = ring[wl][sm][v][lane];
                         }
-                        finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, ivol, hx * g.h[1][rf + 1]);
+                        finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, Dc, ivol, hx * g.h[1][rf + 1]);
                     }
 #pragma unroll
                     for (int v = 0; v < NV; v++) cy[v][lane] = G[v] * Sy;

@@ -87,6 +87,9 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     // FP32-residual stages 2 and 3 with the increment form (see stage_tile)
     constexpr bool INC_F2 = MODE == MODE_F2 && sizeof(TOut) == 4, INC_S3 = MODE == MODE_S3 && sizeof(TIn) == 4;
     const bool inc = (INC_F2 && a.dout != nullptr) || (INC_S3 && a.din != nullptr);
+    // the centre L(U^n) (stage 2) or increment (stage 3) is loaded with the centre U^n
+    using IncT = typename std::conditional<MODE == MODE_F2, TOut, TIn>::type;
+    const IncT* incsrc = (MODE == MODE_F2) ? (const IncT*)a.lin : (const IncT*)a.din;
 
     // Stencil input at (xg, row r, plane kk), in two steps: the loads (issue_cell), and the
     // value formed from them with its ghost image (complete_cell).  Prefetched cells are
@@ -144,6 +147,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
     R Zx[NV];               // centre row of the z-neighbour no wavefront holds (edge wavefronts)
     RawCell Zq;             // its prefetch
     double Nc[NV];          // U^n of the row finalized in this iteration (SSP stages 2, 3)
+    IncT Dc[NV];            // its L(U^n) or increment (FP32-residual stages 2, 3)
     R ytop[NV];             // +y face state of the previous row
     // carried to the next row in LDS, per wavefront and lane (no barrier needed): the y-flux
     // through the pending row's bottom face and the x and z parts of its residual
@@ -183,6 +187,10 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                 const unsigned offn = (unsigned(ks) * unsigned(ny) + unsigned(rc - 1)) * unsigned(nx) + unsigned(xs);
 #pragma unroll
                 for (int v = 0; v < NV; v++) Nc[v] = ld_plane(a.un, P, v, offn);
+                if (inc) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) Dc[v] = ld_plane(incsrc, P, v, offn);
+                }
             }
         }
         int rcs, rcside;
@@ -336,13 +344,13 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
                         } else if (inc && MODE == MODE_F2) {
                             // FP32-residual stage 2: the increment dt/4 (L(U^n) + L(U1)) in float
                             // beside U2 (stage 3 accumulates U^n + 2/3 (D + dt L(U2)) in double)
-                            const double d = 0.25 * dt * (double(ld_plane(lin, P, v, off)) + double(res) * ivol);
+                            const double d = 0.25 * dt * (double(Dc[v]) + double(res) * ivol);
                             if (out_lane) {
                                 st_plane((TOut*)a.dout, P, v, off, TOut(d));
                                 st_plane(uo, P, v, off, TOut(Nc[v] + d));
                             }
                         } else if (inc && MODE == MODE_S3) {
-                            const double d = double(ld_plane((const TIn*)a.din, P, v, off));
+                            const double d = double(Dc[v]);
                             const double q = Nc[v] + (2.0 / 3.0) * (d + dt * (double(res) * ivol));
                             if (out_lane) st_plane(uo, P, v, off, TOut(q));
                         } else {

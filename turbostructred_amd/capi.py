@@ -391,7 +391,7 @@ class Batch:
     """Owner of one ts_batch: many independent solves advanced together (continuous batching)."""
 
     def __init__(self, hbm_budget_bytes=8 << 30, max_active=1 << 20, use_graphs=True, device=0,
-                 autotune_tiles=True):
+                 autotune_tiles=False):
         self.lib = load()
         bc = BatchConfig()
         bc.device = device

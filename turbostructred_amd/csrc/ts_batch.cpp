@@ -113,6 +113,7 @@ struct Problem {
     ts_step_info info{};
     int status = 0;  // 0 queued, 1 resident (or being admitted), 2 done
     int error = 0;
+    int steps_seen = 0;  // steps done as of the last pass (the autotuner's work count)
 };
 
 // Tile-shape autotuning of a family (ts_batch_config.autotune_tiles): for each load bucket
@@ -156,7 +157,7 @@ struct Family {
     int bucket = -1, cand = -1;  // cand -1: the load rule
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool timed = false;
-    double pass_cell_steps = 0;
+    double pass_cell_steps = 0;  // cell-steps the timed pass actually advanced
 };
 
 }  // namespace
@@ -787,7 +788,6 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
             if (timed) {
                 f.timed = true;
                 f.pass_cell_steps = 0;
-                for (const Problem* q : f.members) f.pass_cell_steps += double(q->ncell) * n_steps;
             }
         }
         // control blocks of every resident solve back to the host
@@ -814,6 +814,15 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
         BCHK(hipGetLastError());
         BCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, nres * sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
         BCHK(hipStreamSynchronize(b->stream));
+        // the steps each solve advanced in this pass (a solve that stops part-way does fewer than
+        // n_steps), summed per family for the autotuner
+        for (int i = 0; i < nres; i++) {
+            Problem& p = *resident[i];
+            const int d = b->h_ctl[i].steps_done - p.steps_seen;
+            p.steps_seen = b->h_ctl[i].steps_done;
+            Family& f = b->families[p.family];
+            if (f.timed) f.pass_cell_steps += double(p.ncell) * d;
+        }
         for (size_t fi = 0; fi < nfam; fi++) {
             int r = credit_trial(b, b->families[fi]);
             if (r) return r;

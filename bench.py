@@ -486,9 +486,11 @@ def traffic_from_profiles(kernel):
     path = os.path.join(ROOT, "profiles", "c2_2048_fp64_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    found = None
-    for line in open(path):  # the traffic table comes last; its last column is bytes per launch
-        if line.startswith(f"| {kernel} |"):
+    found, in_table = None, False
+    for line in open(path):  # the table whose last column is "HBM bytes per launch"
+        if line.startswith("| kernel |"):
+            in_table = line.rstrip().rstrip("|").strip().endswith("HBM bytes per launch")
+        elif in_table and line.startswith(f"| {kernel} |"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:
                 found = float(cells[-1])

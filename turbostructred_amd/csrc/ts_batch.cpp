@@ -37,6 +37,18 @@ static_assert(sizeof(StepCtl) % 8 == 0, "control block is copied as 8-byte words
 
 // One launch packs the control blocks of every resident solve (scattered over the arena) into a
 // contiguous buffer, so that they come back to the host in a single copy.
+// descriptors uploaded contiguously, then placed at their slots: one block per descriptor
+static_assert(sizeof(KArgs) % 8 == 0, "descriptors are copied in 8-byte words");
+__global__ void __launch_bounds__(64) scatter_desc(const KArgs* __restrict__ src, const int* __restrict__ dst_index,
+                                                   int n, KArgs* __restrict__ dst) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    constexpr int W = sizeof(KArgs) / 8;
+    const unsigned long long* s = reinterpret_cast<const unsigned long long*>(src + i);
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(dst + dst_index[i]);
+    for (int w = threadIdx.x; w < W; w += blockDim.x) d[w] = s[w];
+}
+
 __global__ void __launch_bounds__(256) gather_ctl(StepCtl* const* src, StepCtl* dst, int n) {
     constexpr int W = sizeof(StepCtl) / 8;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,6 +126,8 @@ struct Problem {
     int status = 0;  // 0 queued, 1 resident (or being admitted), 2 done
     int error = 0;
     int steps_seen = 0;  // steps done as of the last pass (the autotuner's work count)
+    int slot = -1;       // index of its descriptors in its family's descriptor blocks
+    bool uploaded = false;  // its descriptors are in place for the family's current tiling
 };
 
 // Tile-shape autotuning of a family (ts_batch_config.autotune_tiles): for each load bucket
@@ -152,6 +166,16 @@ struct Family {
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    // descriptor slots: a resident problem keeps its slot until it retires, so a membership change
+    // uploads only the new problems' descriptors (all of them when the tiling or the arrays change)
+    std::vector<int> free_slots;
+    int slot_end = 0;        // slots handed out so far: [0, slot_end)
+    int param = -1;          // tiling parameter chosen by the last retile (rows, or strips per item)
+    int uploaded_param = -1; // the one the uploaded descriptors carry
+    KArgs* d_fresh = nullptr;  // staging for new descriptors and their destination indices
+    int* d_fresh_idx = nullptr;
+    size_t cap_fresh = 0;
+    int h_ntiles = 0;
     // autotuning: trials per load bucket, the tiling in use, and the timing of the current pass
     std::map<int, LoadTuning> tuning;
     int bucket = -1, cand = -1;  // cand -1: the load rule
@@ -425,6 +449,7 @@ void retile(ts_batch* b, Family& f) {
             }
         }
         if (f.cand >= 0) G = kCand1D[f.cand];
+        f.param = G;
         for (Problem* q : f.members) {
             Problem& p = *q;
             p.desc.ty = G;
@@ -446,6 +471,7 @@ void retile(ts_batch* b, Family& f) {
         }
     }
     if (f.cand >= 0) ty = kCand2D[f.cand];
+    f.param = ty;
     for (Problem* q : f.members) {
         Problem& p = *q;
         p.desc.ty = ty;
@@ -456,18 +482,19 @@ void retile(ts_batch* b, Family& f) {
 
 int rebuild_tables(ts_batch* b, Family& f) {
     retile(b, f);
-    size_t np = f.members.size();
     size_t nt = 0;
     for (const Problem* q : f.members) nt += q->ntiles;
     bool realloc = false;
-    if (np > f.cap_desc) {  // cap_desc descriptors per kernel, three kernels
+    if ((size_t)f.slot_end > f.cap_desc) {  // cap_desc descriptors per kernel, three kernels
         if (f.d_desc) BCHK(hipFree(f.d_desc));
-        f.cap_desc = std::max<size_t>(np * 2, 64);
+        f.d_desc = nullptr;
+        f.cap_desc = std::max<size_t>((size_t)f.slot_end * 2, 64);
         BCHK(hipMalloc(&f.d_desc, 3 * f.cap_desc * sizeof(KArgs)));
         realloc = true;
     }
     if (nt > f.cap_table) {
         if (f.d_table) BCHK(hipFree(f.d_table));
+        f.d_table = nullptr;
         f.cap_table = std::max<size_t>(nt * 2, 4096);
         BCHK(hipMalloc(&f.d_table, f.cap_table * sizeof(int2)));
         realloc = true;
@@ -480,21 +507,46 @@ int rebuild_tables(ts_batch* b, Family& f) {
         (void)hipGraphExecDestroy(f.graph);
         f.graph = nullptr;
     }
-    // descriptors: one per (kernel, problem), kernel k's block starting at k * cap_desc so that
-    // the launch arguments stay fixed while the resident set changes
-    std::vector<KArgs> desc(3 * f.cap_desc);
+    // descriptors: kernel k's block starts at k * cap_desc, so that the launch arguments stay
+    // fixed while the resident set changes; only problems whose descriptors are not in place for
+    // the current tiling are uploaded, contiguously, and scattered to their slots on the device
+    const bool all = realloc || f.param != f.uploaded_param;
+    std::vector<KArgs> fresh;
+    std::vector<int> fresh_idx;
+    for (Problem* q : f.members) {
+        if (!all && q->uploaded) continue;
+        for (int k = 0; k < 3; k++) {
+            fresh.push_back(stage_desc(*q, k, b->d_arena));
+            fresh_idx.push_back(int(k * f.cap_desc) + q->slot);
+        }
+        q->uploaded = true;
+    }
+    f.uploaded_param = f.param;
+    if (!fresh.empty()) {
+        if (fresh.size() > f.cap_fresh) {
+            if (f.d_fresh) BCHK(hipFree(f.d_fresh));
+            if (f.d_fresh_idx) BCHK(hipFree(f.d_fresh_idx));
+            f.d_fresh = nullptr;
+            f.d_fresh_idx = nullptr;
+            f.cap_fresh = std::max<size_t>(fresh.size() * 2, 192);
+            BCHK(hipMalloc(&f.d_fresh, f.cap_fresh * sizeof(KArgs)));
+            BCHK(hipMalloc(&f.d_fresh_idx, f.cap_fresh * sizeof(int)));
+        }
+        BCHK(hipMemcpyAsync(f.d_fresh, fresh.data(), fresh.size() * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
+        BCHK(hipMemcpyAsync(f.d_fresh_idx, fresh_idx.data(), fresh_idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                            b->stream));
+        hipLaunchKernelGGL(scatter_desc, dim3((unsigned)fresh.size()), dim3(64), 0, b->stream, f.d_fresh,
+                           f.d_fresh_idx, (int)fresh.size(), f.d_desc);
+        BCHK(hipGetLastError());
+    }
     std::vector<int2> table;
     table.reserve(nt);
-    for (size_t i = 0; i < np; i++) {
-        const Problem& p = *f.members[i];
-        for (int k = 0; k < 3; k++) desc[k * f.cap_desc + i] = stage_desc(p, k, b->d_arena);
-        for (int t = 0; t < p.ntiles; t++) table.push_back(make_int2((int)i, t));
-    }
-    BCHK(hipMemcpyAsync(f.d_desc, desc.data(), desc.size() * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
+    for (const Problem* q : f.members)
+        for (int t = 0; t < q->ntiles; t++) table.push_back(make_int2(q->slot, t));
     if (nt) BCHK(hipMemcpyAsync(f.d_table, table.data(), nt * sizeof(int2), hipMemcpyHostToDevice, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));  // descriptor / table sources are local vectors
-    int ntiles = (int)nt;
-    BCHK(hipMemcpy(f.d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice));
+    f.h_ntiles = (int)nt;
+    BCHK(hipMemcpyAsync(f.d_ntiles, &f.h_ntiles, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));  // the copy sources are local vectors
     f.dirty = false;
     return TS_OK;
 }
@@ -647,6 +699,8 @@ int ts_batch_destroy(ts_batch* b) {
         if (f.d_desc) (void)hipFree(f.d_desc);
         if (f.d_table) (void)hipFree(f.d_table);
         if (f.d_ntiles) (void)hipFree(f.d_ntiles);
+        if (f.d_fresh) (void)hipFree(f.d_fresh);
+        if (f.d_fresh_idx) (void)hipFree(f.d_fresh_idx);
         for (hipEvent_t e : f.ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -767,7 +821,15 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
                 p.error = 1;
                 continue;
             }
-            b->families[p.family].members.push_back(&p);
+            Family& fam = b->families[p.family];
+            if (fam.free_slots.empty()) {
+                p.slot = fam.slot_end++;
+            } else {
+                p.slot = fam.free_slots.back();
+                fam.free_slots.pop_back();
+            }
+            p.uploaded = false;
+            fam.members.push_back(&p);
             b->families[p.family].dirty = true;
         }
         nfam = b->families.size();
@@ -843,6 +905,7 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
             Problem& p = *resident[i];
             Family& f = b->families[p.family];
             f.members.erase(std::find(f.members.begin(), f.members.end(), &p));
+            f.free_slots.push_back(p.slot);
             f.dirty = true;
             finish_retire(b, p, b->h_ctl[i]);
         }

@@ -211,7 +211,8 @@ struct ts_batch {
     // capacity kMaxFamilies is reserved at creation and never exceeded: a submit that adds a
     // family (under the lock) then never moves the ones the server thread works on without it
     std::vector<Family> families;
-    std::vector<double> stage;  // host staging for state conversion
+    double* d_stage = nullptr;  // device staging of AoS states for the layout conversion
+    size_t stage_cap = 0;       // doubles
     StepCtl* h_ctl = nullptr;   // pinned, one per resident problem
     StepCtl** h_ptr = nullptr;  // pinned, device address of each resident problem's control block
     StepCtl* d_ctl = nullptr;   // packed copies of the resident control blocks
@@ -291,6 +292,17 @@ int family_of(ts_batch* b, const ts_config& c) {
     return (int)b->families.size() - 1;
 }
 
+// the device staging buffer holds at least `doubles` values (only the pass thread uses it)
+int ensure_stage(ts_batch* b, size_t doubles) {
+    if (doubles <= b->stage_cap) return TS_OK;
+    if (b->d_stage) BCHK(hipFree(b->d_stage));
+    b->d_stage = nullptr;
+    b->stage_cap = 0;
+    BCHK(hipMalloc((void**)&b->d_stage, doubles * sizeof(double)));
+    b->stage_cap = doubles;
+    return TS_OK;
+}
+
 // upload geometry/state/control of a problem into its arena block and fill its descriptor
 int admit(ts_batch* b, Problem& p) {
     const ts_config& c = p.cfg;
@@ -353,19 +365,12 @@ int admit(ts_batch* b, Problem& p) {
     }
     for (int s = 0; s < 6; s++) g.side_marker[s] = c.side_marker[s];
     BCHK(hipMemcpyAsync(dgeo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, b->stream));
-    // state: AoS with ghosts -> SoA interior
+    // state: the AoS array with ghosts crosses as it is and is converted on the device
     size_t n = p.ncell;
-    if (b->stage.size() < n * 5) b->stage.resize(n * 5);
-    int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
-    size_t nxa = c.nx + 2, nya = c.ny + 2 * gy;
-    for (int k = 0; k < c.nz; k++)
-        for (int j = 0; j < c.ny; j++)
-            for (int i = 0; i < c.nx; i++) {
-                size_t src = (((size_t)(k + gz) * nya + (j + gy)) * nxa + (i + 1)) * 5;
-                size_t d = ((size_t)k * c.ny + j) * c.nx + i;
-                for (int v = 0; v < 5; v++) b->stage[v * n + d] = p.init[src + v];
-            }
-    BCHK(hipMemcpyAsync(base + L.u0, b->stage.data(), n * 5 * 8, hipMemcpyHostToDevice, b->stream));
+    int r = ensure_stage(b, p.init.size());
+    if (r) return r;
+    BCHK(hipMemcpyAsync(b->d_stage, p.init.data(), p.init.size() * 8, hipMemcpyHostToDevice, b->stream));
+    BCHK(tsr::device_aos_to_soa(b->d_stage, (double*)(base + L.u0), c.ndims, p.n, b->stream));
     BCHK(hipStreamSynchronize(b->stream));  // the staging buffer is reused
     p.init.clear();
     p.init.shrink_to_fit();
@@ -622,31 +627,21 @@ int credit_trial(ts_batch* b, Family& f) {
     return TS_OK;
 }
 
-// copy a finished problem's state back to the host: the copy is issued here (into p.init, which
-// is no longer needed after admission) and finish_retire converts it once the stream has synced
-int start_retire(ts_batch* b, Problem& p) {
+// copy a finished problem's state back to the host in the reference AoS layout (ghosts zero):
+// converted on the device in the staging buffer, then one copy.  Needs no lock: the problem is
+// still resident, so poll does not read it.
+int retire_state(ts_batch* b, Problem& p) {
     Layout L = layout_of(p.cfg);
-    p.init.resize(p.ncell * 5);
-    BCHK(hipMemcpyAsync(p.init.data(), b->d_arena + p.off + L.u0, p.ncell * 5 * 8, hipMemcpyDeviceToHost, b->stream));
-    return TS_OK;
-}
-
-// SoA interior -> reference AoS layout (ghosts zero); needs no lock (the problem is still resident,
-// so poll does not read it)
-void convert_retired(Problem& p) {
-    const ts_config& c = p.cfg;
-    const size_t n = p.ncell;
-    int gy = c.ndims > 1 ? 1 : 0, gz = c.ndims > 2 ? 1 : 0;
-    size_t nxa = c.nx + 2, nya = c.ny + 2 * gy, nza = c.nz + 2 * gz;
-    p.result.assign(nxa * nya * nza * 5, 0.0);
-    for (int k = 0; k < c.nz; k++)
-        for (int j = 0; j < c.ny; j++)
-            for (int i = 0; i < c.nx; i++) {
-                size_t dst = (((size_t)(k + gz) * nya + (j + gy)) * nxa + (i + 1)) * 5;
-                size_t s = ((size_t)k * c.ny + j) * c.nx + i;
-                for (int v = 0; v < 5; v++) p.result[dst + v] = p.init[v * n + s];
-            }
+    const size_t na = tsr::aos_doubles(p.cfg.ndims, p.n);
+    int r = ensure_stage(b, na);
+    if (r) return r;
+    p.result.resize(na);
+    BCHK(hipMemsetAsync(b->d_stage, 0, na * sizeof(double), b->stream));
+    BCHK(tsr::device_soa_to_aos((const double*)(b->d_arena + p.off + L.u0), b->d_stage, p.cfg.ndims, p.n, b->stream));
+    BCHK(hipMemcpyAsync(p.result.data(), b->d_stage, na * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));  // the staging buffer is reused
     std::vector<double>().swap(p.init);
+    return TS_OK;
 }
 
 // control fields -> step info, marks the problem done and frees its block (under the lock)
@@ -709,6 +704,7 @@ int ts_batch_destroy(ts_batch* b) {
     if (b->d_ctl) (void)hipFree(b->d_ctl);
     if (b->d_ptr) (void)hipFree(b->d_ptr);
     if (b->d_arena) (void)hipFree(b->d_arena);
+    if (b->d_stage) (void)hipFree(b->d_stage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TS_OK;
@@ -893,13 +889,11 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
         for (int i = 0; i < nres; i++) {
             const StepCtl& c = b->h_ctl[i];
             if (c.active && !c.error) continue;
-            int r = start_retire(b, *resident[i]);
+            int r = retire_state(b, *resident[i]);
             if (r) return r;
             fin.push_back(i);
         }
-        if (!fin.empty()) BCHK(hipStreamSynchronize(b->stream));
-        // the layout conversion of the finished states happens before the lock is taken
-        for (int i : fin) convert_retired(*resident[i]);
+        // the finished states are on the host before the lock is taken
         std::lock_guard<std::mutex> lk(b->mu);
         for (int i : fin) {
             Problem& p = *resident[i];

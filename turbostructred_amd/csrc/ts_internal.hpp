@@ -72,7 +72,7 @@ struct ts_ctx {
     int graph_steps[2] = {0, 0};
     bool use_graphs = true;
 
-    std::vector<double> host_stage;  // SoA staging for uploads / downloads
+    double* d_aos = nullptr;  // device copy of the reference AoS state (uploads / downloads)
     size_t device_bytes = 0;
 
     // Borrowed-memory mode (ts_step_noalloc): device buffers are carved out of `arena` (a fixed
@@ -101,12 +101,16 @@ void host_bc_image(const ts_bc_config& bc, double g, int axis, const double* u, 
 int tuned_rows_per_tile(const ts_config& c, long long slots = 0);
 // the calling thread's error text for entry points without a context (ts_last_error(NULL))
 void set_thread_error(const std::string& m);
+// the reference AoS layout (5 values per cell, ghost layers on active axes) <-> SoA interior
+// planes, on the device; the AoS ghost entries are neither read nor written
+size_t aos_doubles(int nd, const int n[3]);
+hipError_t device_aos_to_soa(const double* d_aos, double* d_soa, int nd, const int n[3], hipStream_t stream);
+hipError_t device_soa_to_aos(const double* d_soa, double* d_aos, int nd, const int n[3], hipStream_t stream);
 }  // namespace tsr
 
 size_t ts_state_size_cfg(const ts_config* c);
 int init_device_ctx(ts_ctx* c, const ts_config* cfg);
 void destroy_device_ctx(ts_ctx* c);
-void aos_to_soa(const ts_ctx* c, const double* aos, double* soa);
-void soa_to_aos(const ts_ctx* c, const double* soa, double* aos);
+void fill_ghost_images(const ts_ctx* c, double* aos);
 int get_cells(ts_ctx* c, const int* ijk, size_t n, double* out);
 int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap);

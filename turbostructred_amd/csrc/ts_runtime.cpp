@@ -249,13 +249,28 @@ size_t ts_state_size(const ts_ctx* c) {
     return nxa * nya * nza * 5;
 }
 
+// The caller's AoS array crosses PCIe as it is; the layout conversion runs on the device, in a
+// device copy of the AoS array allocated on first use.
+static int ensure_aos(ts_ctx* c) {
+    if (c->d_aos) return TS_OK;
+    const size_t bytes = ts_state_size(c) * sizeof(double);
+    hipError_t e = hipMalloc((void**)&c->d_aos, bytes);
+    if (e != hipSuccess) {
+        c->d_aos = nullptr;
+        return ctx_fail(c, e == hipErrorOutOfMemory ? TS_ERR_NOMEM : TS_ERR_HIP, tsr::hip_msg(e, "hipMalloc"));
+    }
+    c->device_bytes += bytes;
+    return TS_OK;
+}
+
 int ts_set_state(ts_ctx* c, const double* aos, size_t n) {
     if (!c || !aos) return TS_ERR_INVALID;
     if (n != ts_state_size(c)) return ctx_fail(c, TS_ERR_INVALID, "state size mismatch");
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    aos_to_soa(c, aos, c->host_stage.data());
-    HIPCHK(c, hipMemcpyAsync(c->dU[0], c->host_stage.data(), c->ncell * 5 * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream));
+    int r = ensure_aos(c);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_aos, aos, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, tsr::device_aos_to_soa(c->d_aos, (double*)c->dU[0], c->nd, c->n, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = 0;
     return TS_OK;
@@ -265,10 +280,15 @@ int ts_get_state(ts_ctx* c, double* aos, size_t n) {
     if (!c || !aos) return TS_ERR_INVALID;
     if (n != ts_state_size(c)) return ctx_fail(c, TS_ERR_INVALID, "state size mismatch");
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    HIPCHK(c, hipMemcpyAsync(c->host_stage.data(), c->dU[c->cur], c->ncell * 5 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+    int r = ensure_aos(c);
+    if (r) return r;
+    // interior cells converted on the device into a zeroed AoS array; the host then writes the
+    // ghost layers' images
+    HIPCHK(c, hipMemsetAsync(c->d_aos, 0, n * sizeof(double), c->stream));
+    HIPCHK(c, tsr::device_soa_to_aos((const double*)c->dU[c->cur], c->d_aos, c->nd, c->n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(aos, c->d_aos, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    soa_to_aos(c, c->host_stage.data(), aos);
+    fill_ghost_images(c, aos);
     return TS_OK;
 }
 

@@ -235,7 +235,6 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->block_dim = dim3(64 * ZW);
         c->grid_dim = dim3(c->ntiles);
     }
-    if (!c->arena) c->host_stage.resize(c->ncell * 5);
     const char* sep = getenv("TS_SEPARATE_DT");
     c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
     c->euler1d_wg = k.rk_stages == 1 && c->nd == 1 && k.precision == TS_FP64 && !c->vis && c->ks.euler1d_chunk &&
@@ -253,35 +252,17 @@ void destroy_device_ctx(ts_ctx* c) {
     if (c->rn_part) (void)hipFree(c->rn_part);
     if (c->d_cells) (void)hipFree(c->d_cells);
     if (c->d_cell_vals) (void)hipFree(c->d_cell_vals);
+    if (c->d_aos) (void)hipFree(c->d_aos);
     if (c->hctl && c->hctl != &c->hctl_local) (void)hipHostFree(c->hctl);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream && !c->arena) (void)hipStreamDestroy(c->stream);
 }
 
-void aos_to_soa(const ts_ctx* c, const double* aos, double* soa) {
-    int gy = c->nd > 1 ? 1 : 0, gz = c->nd > 2 ? 1 : 0;
-    size_t P = c->ncell;
-    for (int k = 0; k < c->n[2]; k++)
-        for (int j = 0; j < c->n[1]; j++)
-            for (int i = 0; i < c->n[0]; i++) {
-                size_t s = gidx(c, i + 1, j + gy, k + gz) * 5;
-                size_t d = ((size_t)k * c->n[1] + j) * c->n[0] + i;
-                for (int v = 0; v < 5; v++) soa[v * P + d] = aos[s + v];
-            }
-}
-
-void soa_to_aos(const ts_ctx* c, const double* soa, double* aos) {
-    size_t total = ts_state_size(c);
-    std::fill(aos, aos + total, 0.0);
+// The ghost layers of an AoS state whose interior is in place and whose ghost entries are zero:
+// periodic images or boundary-condition images of the interior (kernel.h:334-915), one face
+// layer per active axis (edge and corner ghosts stay zero).
+void fill_ghost_images(const ts_ctx* c, double* aos) {
     int g[3] = {1, c->nd > 1 ? 1 : 0, c->nd > 2 ? 1 : 0};
-    size_t P = c->ncell;
-    for (int k = 0; k < c->n[2]; k++)
-        for (int j = 0; j < c->n[1]; j++)
-            for (int i = 0; i < c->n[0]; i++) {
-                size_t d = gidx(c, i + g[0], j + g[1], k + g[2]) * 5;
-                size_t s = ((size_t)k * c->n[1] + j) * c->n[0] + i;
-                for (int v = 0; v < 5; v++) aos[d + v] = soa[v * P + s];
-            }
     // ghost layers: periodic images or boundary-condition images (kernel.h:334-915)
     for (int a = 0; a < c->nd; a++) {
         int ja = a == 0 ? 1 : 0, ka = a == 2 ? 1 : 2;

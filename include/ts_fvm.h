@@ -133,8 +133,8 @@ typedef struct ts_ctx ts_ctx;
 int ts_create(const ts_config* cfg, ts_ctx** out);
 int ts_destroy(ts_ctx* ctx);
 /* Message of the context's last failure.  ts_last_error(NULL) returns the message of the calling
-   thread's last failed call that has no context to hold it (ts_create, ts_step_noalloc,
-   ts_eos_eval). */
+   thread's last failed call that has no context to hold it (ts_create, ts_batch_create,
+   ts_step_noalloc, ts_eos_eval). */
 const char* ts_last_error(const ts_ctx* ctx);
 
 /* Number of doubles of the reference state array (nXAll*nYAll*nZAll*5, kernel.h:240). */

@@ -127,6 +127,7 @@ struct Problem {
     int error = 0;
     int steps_seen = 0;  // steps done as of the last pass (the autotuner's work count)
     int slot = -1;       // index of its descriptors in its family's descriptor blocks
+    StepCtl final_ctl{};  // its control block when it finished (published with its state)
     bool uploaded = false;  // its descriptors are in place for the family's current tiling
 };
 
@@ -211,8 +212,14 @@ struct ts_batch {
     // capacity kMaxFamilies is reserved at creation and never exceeded: a submit that adds a
     // family (under the lock) then never moves the ones the server thread works on without it
     std::vector<Family> families;
-    double* d_stage = nullptr;  // device staging of AoS states for the layout conversion
+    double* d_stage = nullptr;  // device staging of AoS states for the layout conversion (uploads)
     size_t stage_cap = 0;       // doubles
+    // Finished solves leave through a second stream: their states are converted and copied out
+    // while the next pass's kernels run, and published (done, memory released) at that pass.
+    hipStream_t cstream = nullptr;
+    double* d_rstage = nullptr;  // staging of the outgoing AoS states, one region per solve
+    size_t rstage_cap = 0;
+    std::vector<Problem*> retiring;
     StepCtl* h_ctl = nullptr;   // pinned, one per resident problem
     StepCtl** h_ptr = nullptr;  // pinned, device address of each resident problem's control block
     StepCtl* d_ctl = nullptr;   // packed copies of the resident control blocks
@@ -627,20 +634,17 @@ int credit_trial(ts_batch* b, Family& f) {
     return TS_OK;
 }
 
-// copy a finished problem's state back to the host in the reference AoS layout (ghosts zero):
-// converted on the device in the staging buffer, then one copy.  Needs no lock: the problem is
-// still resident, so poll does not read it.
-int retire_state(ts_batch* b, Problem& p) {
+// Issue, on the copy stream, the conversion of a finished problem's state to the reference AoS
+// layout (ghosts zero) in its region of the outgoing staging buffer and the copy to the host.
+// Needs no lock: the problem is out of its family but not yet published, so poll does not read it.
+int issue_retire(ts_batch* b, Problem& p, size_t region) {
     Layout L = layout_of(p.cfg);
     const size_t na = tsr::aos_doubles(p.cfg.ndims, p.n);
-    int r = ensure_stage(b, na);
-    if (r) return r;
+    double* st = b->d_rstage + region;
     p.result.resize(na);
-    BCHK(hipMemsetAsync(b->d_stage, 0, na * sizeof(double), b->stream));
-    BCHK(tsr::device_soa_to_aos((const double*)(b->d_arena + p.off + L.u0), b->d_stage, p.cfg.ndims, p.n, b->stream));
-    BCHK(hipMemcpyAsync(p.result.data(), b->d_stage, na * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));  // the staging buffer is reused
-    std::vector<double>().swap(p.init);
+    BCHK(hipMemsetAsync(st, 0, na * sizeof(double), b->cstream));
+    BCHK(tsr::device_soa_to_aos((const double*)(b->d_arena + p.off + L.u0), st, p.cfg.ndims, p.n, b->cstream));
+    BCHK(hipMemcpyAsync(p.result.data(), st, na * sizeof(double), hipMemcpyDeviceToHost, b->cstream));
     return TS_OK;
 }
 
@@ -671,9 +675,11 @@ int ts_batch_create(const ts_batch_config* bcfg, ts_batch** out) {
     hipError_t e;
     if ((e = hipSetDevice(b->bc.device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_arena, b->bc.hbm_budget_bytes)) != hipSuccess) {
-        fprintf(stderr, "ts_batch_create: %s\n", hipGetErrorString(e));
+        tsr::set_thread_error("ts_batch_create: " + tsr::hip_msg(e, "device set-up"));
         if (b->stream) (void)hipStreamDestroy(b->stream);
+        if (b->cstream) (void)hipStreamDestroy(b->cstream);
         delete b;
         return e == hipErrorOutOfMemory ? TS_ERR_NOMEM : TS_ERR_HIP;
     }
@@ -689,6 +695,7 @@ int ts_batch_destroy(ts_batch* b) {
     ts_batch_stop_server(b);
     (void)hipSetDevice(b->bc.device);
     (void)hipStreamSynchronize(b->stream);
+    if (b->cstream) (void)hipStreamSynchronize(b->cstream);
     for (Family& f : b->families) {
         if (f.graph) (void)hipGraphExecDestroy(f.graph);
         if (f.d_desc) (void)hipFree(f.d_desc);
@@ -705,7 +712,9 @@ int ts_batch_destroy(ts_batch* b) {
     if (b->d_ptr) (void)hipFree(b->d_ptr);
     if (b->d_arena) (void)hipFree(b->d_arena);
     if (b->d_stage) (void)hipFree(b->d_stage);
+    if (b->d_rstage) (void)hipFree(b->d_rstage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->cstream) (void)hipStreamDestroy(b->cstream);
     delete b;
     return TS_OK;
 }
@@ -779,6 +788,19 @@ int ts_batch_submit(ts_batch* b, const ts_config* cfg, const double* init_state,
     return TS_OK;
 }
 
+// wait for the outgoing copies in flight and publish those solves (done, memory released)
+static int complete_retiring(ts_batch* b) {
+    if (b->retiring.empty()) return TS_OK;
+    BCHK(hipStreamSynchronize(b->cstream));
+    std::lock_guard<std::mutex> lk(b->mu);
+    for (Problem* p : b->retiring) {
+        std::vector<double>().swap(p->init);
+        finish_retire(b, *p, p->final_ctl);
+    }
+    b->retiring.clear();
+    return TS_OK;
+}
+
 // one admission + advance pass; the caller is either the server thread or, when no server
 // runs, the user through ts_batch_run.  Runs under run_mu.
 static int batch_run(ts_batch* b, int n_steps, int* pending) {
@@ -848,6 +870,11 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
                 f.pass_cell_steps = 0;
             }
         }
+        // the previous pass's finished solves, copied out while these kernels run
+        {
+            int r = complete_retiring(b);
+            if (r) return r;
+        }
         // control blocks of every resident solve back to the host
         if (b->h_ctl_cap < resident.size()) {
             BCHK(hipStreamSynchronize(b->stream));
@@ -885,28 +912,60 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
             int r = credit_trial(b, b->families[fi]);
             if (r) return r;
         }
+        // finished solves: their states go out on the copy stream (the kernels that wrote them
+        // have completed: the stream was synchronised above), one staging region each
         std::vector<int> fin;
+        size_t need = 0;
         for (int i = 0; i < nres; i++) {
             const StepCtl& c = b->h_ctl[i];
             if (c.active && !c.error) continue;
-            int r = retire_state(b, *resident[i]);
-            if (r) return r;
             fin.push_back(i);
+            need += tsr::aos_doubles(resident[i]->cfg.ndims, resident[i]->n);
         }
-        // the finished states are on the host before the lock is taken
-        std::lock_guard<std::mutex> lk(b->mu);
-        for (int i : fin) {
-            Problem& p = *resident[i];
-            Family& f = b->families[p.family];
-            f.members.erase(std::find(f.members.begin(), f.members.end(), &p));
-            f.free_slots.push_back(p.slot);
-            f.dirty = true;
-            finish_retire(b, p, b->h_ctl[i]);
+        if (!fin.empty()) {
+            if (need > b->rstage_cap) {  // the copy stream is idle: complete_retiring ran above
+                if (b->d_rstage) BCHK(hipFree(b->d_rstage));
+                b->d_rstage = nullptr;
+                b->rstage_cap = 0;
+                BCHK(hipMalloc((void**)&b->d_rstage, need * 2 * sizeof(double)));
+                b->rstage_cap = need * 2;
+            }
+            size_t region = 0;
+            for (int i : fin) {
+                Problem& p = *resident[i];
+                p.final_ctl = b->h_ctl[i];
+                int r = issue_retire(b, p, region);
+                if (r) return r;
+                region += tsr::aos_doubles(p.cfg.ndims, p.n);
+            }
+            std::lock_guard<std::mutex> lk(b->mu);
+            for (int i : fin) {
+                Problem& p = *resident[i];
+                Family& f = b->families[p.family];
+                f.members.erase(std::find(f.members.begin(), f.members.end(), &p));
+                f.free_slots.push_back(p.slot);
+                f.dirty = true;
+                b->retiring.push_back(&p);
+            }
+        }
+    }
+    // with no solve left to advance, nothing would overlap the outgoing copies: publish them now
+    {
+        bool idle;
+        {
+            std::lock_guard<std::mutex> lk(b->mu);
+            size_t nres = 0;
+            for (Family& f : b->families) nres += f.members.size();
+            idle = nres == 0 || n_steps <= 0;
+        }
+        if (idle) {
+            int r = complete_retiring(b);
+            if (r) return r;
         }
     }
     if (pending) {
         std::lock_guard<std::mutex> lk(b->mu);
-        size_t n = b->queue.size();
+        size_t n = b->queue.size() + b->retiring.size();
         for (Family& f : b->families) n += f.members.size();
         *pending = (int)n;
     }

@@ -442,33 +442,58 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
 }
 
 // ---- exact Riemann solver (always double) -----------------------------------------------------
-// Every power in the exact solver goes through one out-of-line copy of ocml's double pow: inlined
-// at each of its dozen call sites it made the kernels larger than the instruction cache, and
-// the run time was then set by instruction fetch.
-static __device__ __noinline__ double dpow(double x, double y) { return pow(x, y); }
+// Powers of positive bases, x^y = exp(y ln x), in one out-of-line copy: inlined at each of its
+// call sites it made the kernels larger than the instruction cache, and the run time was then
+// set by instruction fetch.  Every base here is positive (pressure ratios, the two-rarefaction
+// guess, the fan's C_K); ln and exp are within an ulp, so the power is within a few ulps for the
+// exponents used (|y ln x| below about 10), against about twice the instructions for a
+// correctly rounded pow.  x = 0 gives 0 and x < 0 NaN, as pow does for these exponents.
+static __device__ __noinline__ double dpow(double x, double y) { return exp(y * log(x)); }
 
-// f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37).
-__device__ inline double fk_dev(double g, double r, double pk, double ak, double p, double& d, double& pz) {
-    if (p > pk) {
-        double A = 2.0 / ((g + 1.0) * r);
-        double B = pk * (g - 1.0) / (g + 1.0);
-        double q = __builtin_sqrt(A / (p + B));
-        d = q * (1.0 - 0.5 * (p - pk) / (B + p));
-        pz = 0.0;
-        return (p - pk) * q;
-    }
-    // rarefaction: one power, (p/pK)^z with z = (g-1)/2g, gives f, f' = (p/pK)^(z-1)/(r aK) and
-    // the tail speed of the fan (the sampler reuses pz)
-    double pr = p / pk;
-    pz = dpow(pr, (g - 1.0) / (2.0 * g));
-    double res = pz - 1.0;
-    res *= 2.0 * ak / (g - 1.0);
-    d = pz / (pr * r * ak);
-    return res;
+// The side-K constants of f_K (GodunovSolverPerfectGasEOS.h:58-97), formed once per face
+// instead of once per Newton iteration.
+struct SideDev {
+    double pk, ak;
+    double A, B;  // 2 / ((g+1) rho_K), p_K (g-1)/(g+1): the shock branch
+    double ipk;   // 1 / p_K
+    double ira;   // 1 / (rho_K a_K)
+    double c2a;   // 2 a_K / (g-1)
+};
+// cA = 2/(g+1), cB = (g-1)/(g+1), c2 = 2/(g-1)
+__device__ inline SideDev side_dev(double cA, double cB, double c2, double r, double pk, double ak) {
+    SideDev s;
+    s.pk = pk;
+    s.ak = ak;
+    s.A = cA * frcp(r);
+    s.B = pk * cB;
+    s.ipk = frcp(pk);
+    s.ira = frcp(r * ak);
+    s.c2a = c2 * ak;
+    return s;
 }
 
+// f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37);
+// z = (g-1)/2g.  The reciprocals and the square root are the refined estimates (a few ulps).
+__device__ inline double fk_dev(const SideDev& s, double z, double p, double& d, double& pz) {
+    if (p > s.pk) {
+        const double ipb = frcp(p + s.B);
+        const double q = fsqrt(s.A * ipb);
+        d = q * (1.0 - 0.5 * (p - s.pk) * ipb);
+        pz = 0.0;
+        return (p - s.pk) * q;
+    }
+    // rarefaction: one power, (p/pK)^z, gives f, f' = (p/pK)^(z-1)/(r aK) and the tail speed of
+    // the fan (the sampler reuses pz)
+    const double pr = p * s.ipk;
+    pz = dpow(pr, z);
+    d = pz * frcp(pr) * s.ira;
+    return (pz - 1.0) * s.c2a;
+}
+
+// Star pressure and velocity, wave speeds and smax; the star densities are formed by the sampler,
+// for the sampled side only.
 struct StarDev {
-    double p, u, rl, rr, SL, SHL, STL, SR, SHR, STR, smax;
+    double p, u, SL, SHL, STL, SR, SHR, STR, smax;
     bool shock_l, shock_r, vacuum;
 };
 
@@ -482,6 +507,9 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
     double al = __builtin_sqrt(g * pl / rl), ar = __builtin_sqrt(g * pr / rr);
     double du = ur - ul;
     s.vacuum = (2.0 / (g - 1.0)) * (al + ar) <= du;
+    const double z = (g - 1.0) / (2.0 * g);
+    const double cA = 2.0 / (g + 1.0), cB = (g - 1.0) / (g + 1.0), c2 = 2.0 / (g - 1.0);
+    const SideDev sl = side_dev(cA, cB, c2, rl, pl, al), sr = side_dev(cA, cB, c2, rr, pr, ar);
     double p = 0.0;
     // Inputs that are not a physical state (zero, negative or NaN density or pressure) give a
     // NaN star pressure straight away.  Such inputs reach the solver on halo lanes, which are
@@ -497,7 +525,6 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         if (pmax / pmin <= 2.0 && pmin <= ppv && ppv <= pmax) {
             p = ppv;
         } else if (ppv < pmin) {
-            double z = (g - 1.0) / (2.0 * g);
             double plz = dpow(pl, z), prz = dpow(pr, z);
             double t = (al + ar - 0.5 * (g - 1.0) * du) / (al / plz + ar / prz);
             p = dpow(t, 1.0 / z);
@@ -512,14 +539,15 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         if (!(p > 0.0)) p = 1e-12 * pmin;
         for (int it = 1; it <= 60; it++) {
             double dl, dr, zl, zr;
-            double fl = fk_dev(g, rl, pl, al, p, dl, zl), fr = fk_dev(g, rr, pr, ar, p, dr, zr);
+            double fl = fk_dev(sl, z, p, dl, zl), fr = fk_dev(sr, z, p, dr, zr);
             pe = p, fle = fl, fre = fr, zle = zl, zre = zr;
             double f = fl + fr + du;
-            double pn = p - f / (dl + dr);
+            double pn = p - fdiv(f, dl + dr);
             if (!(pn > 0.0)) pn = 0.01 * p;
-            double ch = __builtin_fabs(pn - p) / (pn + p);
+            // relative change |pn - p| / (pn + p) <= 1e-15, without the division (pn + p > 0)
+            const bool done = __builtin_fabs(pn - p) <= 1e-15 * (pn + p);
             p = pn;
-            if (ch <= 1e-15) break;
+            if (done) break;
         }
     }
     s.p = p;
@@ -528,22 +556,19 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         f1 = fle, f2 = fre, pzl = zle, pzr = zre;
     } else {
         double d1, d2;
-        f1 = fk_dev(g, rl, pl, al, p, d1, pzl);
-        f2 = fk_dev(g, rr, pr, ar, p, d2, pzr);
+        f1 = fk_dev(sl, z, p, d1, pzl);
+        f2 = fk_dev(sr, z, p, d2, pzr);
     }
     s.u = 0.5 * (ul + ur) + 0.5 * (f2 - f1);
-    double C = (g - 1.0) / (g + 1.0);
     s.smax = 0.0;
     double prl = p / pl;
     s.SL = s.SHL = s.STL = s.SR = s.SHR = s.STR = 0.0;
     if (p > pl) {
         s.shock_l = true;
-        s.rl = (prl + C) / (C * prl + 1.0) * rl;
         s.SL = ul - al * __builtin_sqrt((g + 1) * prl / (2 * g) + (g - 1) / (2 * g));
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SL));
     } else {
         s.shock_l = false;
-        s.rl = rl * dpow(prl, 1.0 / g);
         s.SHL = ul - al;
         s.STL = s.u - al * pzl;  // (p/pL)^((g-1)/2g), from f_L
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHL));
@@ -551,12 +576,10 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
     double prr = p / pr;
     if (p > pr) {
         s.shock_r = true;
-        s.rr = (prr + C) / (C * prr + 1.0) * rr;
         s.SR = ur + ar * __builtin_sqrt((g + 1) * prr / (2 * g) + (g - 1) / (2 * g));
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SR));
     } else {
         s.shock_r = false;
-        s.rr = rr * dpow(prr, 1.0 / g);
         s.SHR = ur + ar;
         s.STR = s.u + ar * pzr;
         s.smax = __builtin_fmax(s.smax, __builtin_fabs(s.SHR));
@@ -589,34 +612,42 @@ __device__ inline void godunov_flux(R gammaR, const R (&ULr)[NV], const R (&URr)
     double ro, u, p;
     const double S = 0.0;
     bool left = st.vacuum ? (S <= 0.5 * (st.STL + st.STR)) : (st.u >= S);
-    if (left) {
-        if (st.shock_l && !st.vacuum) {
-            if (st.SL >= S) { ro = UL[0]; u = uL; p = pL; }
-            else { ro = st.rl; u = st.u; p = st.p; }
-        } else if (st.SHL > S) { ro = UL[0]; u = uL; p = pL; }
-        else if (S > st.STL) {
-            if (st.vacuum) { ro = 0; u = 0; p = 0; } else { ro = st.rl; u = st.u; p = st.p; }
+    // The side of the contact the sample lies on, then one of four cases for that side K:
+    // 0 its initial state, 1 its star state, 2 inside its rarefaction fan, 3 vacuum.  The
+    // conditions are the reference's with the right side mirrored by sgn = -1 (exact), so one
+    // fan evaluation serves both sides.
+    const double sgn = left ? 1.0 : -1.0;
+    const bool shockK = left ? st.shock_l : st.shock_r;
+    const double SK = left ? st.SL : st.SR, SHK = left ? st.SHL : st.SHR, STK = left ? st.STL : st.STR;
+    const double rK = left ? UL[0] : UR[0], uK = left ? uL : uR, pK = left ? pL : pR;
+    int cs;
+    if (shockK && !st.vacuum) cs = (sgn * (SK - S) >= 0.0) ? 0 : 1;
+    else if (sgn * (SHK - S) > 0.0) cs = 0;
+    else if (sgn * (S - STK) > 0.0) cs = st.vacuum ? 3 : 1;
+    else cs = 2;
+    if (cs == 0) { ro = rK; u = uK; p = pK; }
+    else if (cs == 1) {
+        // star density behind a shock (Toro eq. 4.50) or a rarefaction (eq. 4.53)
+        const double prK = st.p / pK;
+        if (shockK) {
+            const double C = (g - 1.0) / (g + 1.0);
+            ro = (prK + C) / (C * prK + 1.0) * rK;
         } else {
-            double al = __builtin_sqrt(g * pL / UL[0]);
-            double CL = 2.0 / (g + 1) + (g - 1) * (uL - S) / ((g + 1) * al);
-            ro = dpow(CL, 2.0 / (g - 1)) * UL[0];
-            u = (al + 0.5 * (g - 1) * uL + S) * (2 / (g + 1));
-            p = dpow(CL, 2.0 * g / (g - 1)) * pL;
+            ro = rK * dpow(prK, 1.0 / g);
         }
-    } else {
-        if (st.shock_r && !st.vacuum) {
-            if (st.SR <= S) { ro = UR[0]; u = uR; p = pR; }
-            else { ro = st.rr; u = st.u; p = st.p; }
-        } else if (st.SHR < S) { ro = UR[0]; u = uR; p = pR; }
-        else if (S < st.STR) {
-            if (st.vacuum) { ro = 0; u = 0; p = 0; } else { ro = st.rr; u = st.u; p = st.p; }
-        } else {
-            double ar = __builtin_sqrt(g * pR / UR[0]);
-            double CR = 2.0 / (g + 1) - (g - 1) * (uR - S) / ((g + 1) * ar);
-            ro = dpow(CR, 2.0 / (g - 1)) * UR[0];
-            u = (-ar + 0.5 * (g - 1) * uR + S) * (2 / (g + 1));
-            p = dpow(CR, 2.0 * g / (g - 1)) * pR;
-        }
+        u = st.u;
+        p = st.p;
+    }
+    else if (cs == 3) { ro = 0; u = 0; p = 0; }
+    else {
+        // inside the fan (Toro eq. 4.56): C^(2/(g-1)) scales the density, and the pressure by
+        // C^(2g/(g-1)) = C^(2/(g-1)) C^2
+        const double aK = __builtin_sqrt(g * pK / rK);
+        const double C = 2.0 / (g + 1) + sgn * ((g - 1) * (uK - S) / ((g + 1) * aK));
+        const double w = dpow(C, 2.0 / (g - 1));
+        ro = w * rK;
+        u = (sgn * aK + 0.5 * (g - 1) * uK + S) * (2 / (g + 1));
+        p = w * C * C * pK;
     }
     const double* vel = left ? vL : vR;
     double Vf[3] = {vel[0], vel[1], vel[2]};

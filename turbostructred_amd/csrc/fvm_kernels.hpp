@@ -250,6 +250,14 @@ __device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
     atomicExch(&ctl->min_bits, ~0ull);
     atomicExch(&ctl->blocks_done, 0u);
 }
+// The same after contributions that each waited for their own atomic minimum to be performed
+// (wave_reduce_finalize): the minimum is read with an atomic at the same coherence point, and
+// no fence is needed.
+__device__ inline void dt_finalize_ordered(StepCtl* ctl, double cfl) {
+    dt_apply(ctl, cfl, bitsd(atomicAdd(&ctl->min_bits, 0ull)));
+    atomicExch(&ctl->min_bits, ~0ull);
+    atomicExch(&ctl->blocks_done, 0u);
+}
 
 // The finalize as a kernel of its own, launched right after a dt pass or fused stage 1 whose
 // wavefronts (3-D: workgroups) only post their minima with non-returning atomics: nothing in the
@@ -268,9 +276,13 @@ __device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, 
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
     if (lane == 0) {
-        atomicMin(&ctl->min_bits, dbits(m));
-        __threadfence();
-        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize(ctl, cfl);
+        // Both are device-scope atomics, performed at one coherence point.  The wavefront waits
+        // for the minimum's returned value before it issues the increment, so the contributor
+        // that counts last finds every minimum already performed.  (A fence in its place wrote
+        // back the L2 for every work item: an agent-scope release on this part.)
+        const unsigned long long prev = atomicMin(&ctl->min_bits, dbits(m));
+        __asm__ volatile("" ::"v"(prev));
+        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize_ordered(ctl, cfl);
     }
 }
 

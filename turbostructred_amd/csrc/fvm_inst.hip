@@ -29,7 +29,7 @@ static void launch(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
 #else
     hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, R, TIn, TOut>), g, b, 0, s, a);
 #endif
-    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
+    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(1024), 0, s, a.ctl, a.g.cfl, a.dt_part, a.ntiles);
 }
 
 // resident workgroups per CU of the single-solve stage kernels (the minimum over the kernels of
@@ -93,7 +93,7 @@ static void launch_vis(const KArgs& a, dim3 g, dim3 b, hipStream_t s) {
 #else
     hipLaunchKernelGGL((fvm_stage<TS_DIM, RC, LM, TS_FLUX, MODE, double, double, double, 1>), g, b, 0, s, a);
 #endif
-    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(64), 0, s, a.ctl, a.g.cfl);
+    if constexpr (ModeTraits<MODE>::sr) hipLaunchKernelGGL(dt_finalize_kernel<0>, dim3(1), dim3(1024), 0, s, a.ctl, a.g.cfl, a.dt_part, a.ntiles);
 }
 
 #if TS_DIM == 1

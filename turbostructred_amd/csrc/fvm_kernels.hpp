@@ -109,6 +109,7 @@ struct KArgs {
     int nstrips;       // strips along x
     int ntiles;        // strips * row blocks
     double* rn_part;   // per-block partial residual norms, 5 per block (MODE_RN)
+    double* dt_part;   // single solves: each tile's minimum of V/SR (dt passes, fused stage 1)
 };
 
 // ---- wave-level helpers -------------------------------------------------------------------------
@@ -243,30 +244,36 @@ __device__ inline void dt_apply(StepCtl* ctl, double cfl, double mn) {
     }
 }
 
-// dt_apply on the minimum the contributors of a launch posted, then re-arms the reduction cells
+// dt_apply on the minimum that the contributors of a batched launch posted with atomics, each
+// having waited for its own atomic minimum to be performed (wave_reduce_finalize): the minimum
+// is read with an atomic at the same coherence point, and no fence is needed.  Re-arms the
+// reduction cells.
 __device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
-    __threadfence();
-    dt_apply(ctl, cfl, bitsd(atomicAdd(&ctl->min_bits, 0ull)));
-    atomicExch(&ctl->min_bits, ~0ull);
-    atomicExch(&ctl->blocks_done, 0u);
-}
-// The same after contributions that each waited for their own atomic minimum to be performed
-// (wave_reduce_finalize): the minimum is read with an atomic at the same coherence point, and
-// no fence is needed.
-__device__ inline void dt_finalize_ordered(StepCtl* ctl, double cfl) {
     dt_apply(ctl, cfl, bitsd(atomicAdd(&ctl->min_bits, 0ull)));
     atomicExch(&ctl->min_bits, ~0ull);
     atomicExch(&ctl->blocks_done, 0u);
 }
 
-// The finalize as a kernel of its own, launched right after a dt pass or fused stage 1 whose
-// wavefronts (3-D: workgroups) only post their minima with non-returning atomics: nothing in the
-// stage kernel waits for a returned counter.  One wavefront; skipped when no step is running.  (A
-// template only so that the translation units including this header share one definition.)
+// Single solves: the step's dt from the `n` per-tile minima of V/SR that a dt pass or fused
+// stage 1 stored, launched right after it.  Plain stores and one workgroup's reduction instead of
+// one same-address atomic per wavefront: those serialise where the wavefronts finish together,
+// at the end of the launch (a 1-D stage 1 with 70k strips spent most of its time in them).  The
+// minimum is exact, so its order does not matter.  Skipped when no step is running.  (A template
+// only so that the translation units including this header share one definition.)
 template <int = 0>
-__global__ void __launch_bounds__(64) dt_finalize_kernel(StepCtl* ctl, double cfl) {
-    if (threadIdx.x != 0 || !ctl->active) return;
-    dt_finalize(ctl, cfl);
+__global__ void __launch_bounds__(1024) dt_finalize_kernel(StepCtl* ctl, double cfl, const double* part, int n) {
+    if (!ctl->active) return;  // the same for every thread: before the barrier
+    double m = __builtin_inf();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = __builtin_fmin(m, part[i]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+    __shared__ double red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < (blockDim.x >> 6); w++) m = __builtin_fmin(m, red[w]);
+        dt_apply(ctl, cfl, m);
+    }
 }
 
 // Wavefront min of V/SR, one atomic min and one counter increment per wavefront with no
@@ -282,7 +289,7 @@ __device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, 
         // back the L2 for every work item: an agent-scope release on this part.)
         const unsigned long long prev = atomicMin(&ctl->min_bits, dbits(m));
         __asm__ volatile("" ::"v"(prev));
-        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize_ordered(ctl, cfl);
+        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize(ctl, cfl);
     }
 }
 
@@ -806,12 +813,11 @@ __global__ void __launch_bounds__(256) fvm_stage(KArgs a) {
     double rn[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, VIS>(a, tile < a.ntiles ? tile : -1, lane, block_min, rn);
     if constexpr (MT::sr) {
-        // post the wavefront's minimum; dt_finalize_kernel turns it into dt after this launch.
-        // (A returned counter per wavefront to find the last one made F1 40% slower at 2048^2:
-        // those atomics serialise at one address.)
+        // store the wavefront's minimum; dt_finalize_kernel turns the tiles' minima into dt after
+        // this launch
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) block_min = __builtin_fmin(block_min, __shfl_xor(block_min, o));
-        if (lane == 0) atomicMin(&ctl->min_bits, dbits(block_min));
+        if (lane == 0) st_plane(a.dt_part, 0, 0, unsigned(tile), block_min);
     }
     if constexpr (MODE == MODE_RN) {
         __shared__ double rred[4 * NV];

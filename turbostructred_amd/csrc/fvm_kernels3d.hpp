@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         }
     }
     if constexpr (MT::sr) {
-        // block minimum, posted with a non-returning atomic; dt_finalize_kernel follows the launch
+        // the workgroup's minimum, stored for its tile; dt_finalize_kernel follows the launch
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) block_min = __builtin_fmin(block_min, __shfl_xor(block_min, o));
         if (lane == 0) red[w] = block_min;
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(64 * ZW) fvm_stage3d(KArgs a) {
         if (threadIdx.x == 0) {
             double m = red[0];
             for (int q = 1; q < ZW; q++) m = __builtin_fmin(m, red[q]);
-            atomicMin(&ctl->min_bits, dbits(m));
+            st_plane(a.dt_part, 0, 0, unsigned(tile), m);
         }
     }
     if constexpr (MODE == MODE_RN) {

@@ -64,6 +64,7 @@ struct ts_ctx {
     size_t cells_cap = 0;
     bool norms = false;         // residual-norm pass after the first kernel of every step
     double* rn_part = nullptr;  // its per-block partial sums
+    double* dt_part = nullptr;  // the dt passes' per-tile minima of V/SR
     double h_rnorm[5] = {0, 0, 0, 0, 0};
     std::vector<hipEvent_t> events;
 

@@ -235,6 +235,8 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->block_dim = dim3(64 * ZW);
         c->grid_dim = dim3(c->ntiles);
     }
+    // one minimum of V/SR per tile from every dt pass and fused stage 1
+    if ((r = dalloc(c, &c->dt_part, (size_t)c->ntiles))) return r;
     const char* sep = getenv("TS_SEPARATE_DT");
     c->fused_dt = k.rk_stages == 3 && !(sep && sep[0] == '1') && c->ks.f1 && c->ks.f2 && !c->vis;
     c->euler1d_wg = k.rk_stages == 1 && c->nd == 1 && k.precision == TS_FP64 && !c->vis && c->ks.euler1d_chunk &&
@@ -313,6 +315,7 @@ void step_launches(const ts_ctx* c, int start_buf, int s, Launch (&L)[MAX_LAUNCH
     a.ntiles = c->ntiles;
     a.un = (const double*)c->dU[0];
     a.rn_part = c->rn_part;
+    a.dt_part = c->dt_part;
     n = 0;
     // residual norms of U^n, after the kernel that decides whether the step runs
     auto norms = [&](const void* un_buf) {

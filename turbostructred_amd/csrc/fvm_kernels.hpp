@@ -171,20 +171,17 @@ __device__ inline int bc_marker(const GridArgs& g, int side, int face_index) {
 // Variable v of an SoA state array at a cell: a wave-uniform 64-bit plane base plus a 32-bit
 // per-lane byte offset -- the global load/store form with a scalar base address, so that each
 // access needs one VGPR of address and no 64-bit address arithmetic.  A plane stays below
-// 4 GiB (checked when a context is created).  The base is taken as a global-memory address
-// explicitly: the batched kernels read their state pointers from descriptors in memory, and
-// as generic pointers those accesses became flat instructions, which also wait on the scalar
-// loads' counter.
-#define TS_GLOBAL __attribute__((address_space(1)))
+// 4 GiB (checked when a context is created).  (Casting the base to the global address space
+// explicitly changed nothing for the 1-D / 2-D stages and made the viscous stages 10-25% slower.)
 template <typename T>
 __device__ __forceinline__ T ld_plane(const T* base, size_t P, int v, unsigned cell) {
-    const TS_GLOBAL char* b = (const TS_GLOBAL char*)(base + (size_t)v * P);
-    return *(const TS_GLOBAL T*)(b + cell * unsigned(sizeof(T)));
+    const char* b = reinterpret_cast<const char*>(base + (size_t)v * P);
+    return *reinterpret_cast<const T*>(b + cell * unsigned(sizeof(T)));
 }
 template <typename T>
 __device__ __forceinline__ void st_plane(T* base, size_t P, int v, unsigned cell, T x) {
-    TS_GLOBAL char* b = (TS_GLOBAL char*)(base + (size_t)v * P);
-    *(TS_GLOBAL T*)(b + cell * unsigned(sizeof(T))) = x;
+    char* b = reinterpret_cast<char*>(base + (size_t)v * P);
+    *reinterpret_cast<T*>(b + cell * unsigned(sizeof(T))) = x;
 }
 
 // The same store with the non-temporal hint, for the 1-D / 2-D stage outputs: each value is
@@ -193,18 +190,8 @@ __device__ __forceinline__ void st_plane(T* base, size_t P, int v, unsigned cell
 // keeps the plain store, which measured faster there.
 template <typename T>
 __device__ __forceinline__ void st_plane_stream(T* base, size_t P, int v, unsigned cell, T x) {
-    TS_GLOBAL char* b = (TS_GLOBAL char*)(base + (size_t)v * P);
-    __builtin_nontemporal_store(x, (TS_GLOBAL T*)(b + cell * unsigned(sizeof(T))));
-}
-
-// Generic-address forms, for state planes that may live in LDS (the one-workgroup 1-D kernel)
-template <typename T>
-__device__ __forceinline__ T ld_plane_any(const T* base, size_t P, int v, unsigned cell) {
-    return base[(size_t)v * P + cell];
-}
-template <typename T>
-__device__ __forceinline__ void st_plane_any(T* base, size_t P, int v, unsigned cell, T x) {
-    base[(size_t)v * P + cell] = x;
+    char* b = reinterpret_cast<char*>(base + (size_t)v * P);
+    __builtin_nontemporal_store(x, reinterpret_cast<T*>(b + cell * unsigned(sizeof(T))));
 }
 
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -419,7 +406,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     TOut* uo = LDS_STATE ? (TOut*)uout_arg : (TOut*)a.uout;
     const size_t P = LDS_STATE ? plane_arg : a.plane;
     auto store = [&](int v, unsigned off, TOut x) {
-        if constexpr (LDS_STATE) st_plane_any(uo, P, v, off, x);
+        if constexpr (LDS_STATE) st_plane(uo, P, v, off, x);
         else st_plane_stream(uo, P, v, off, x);
     };
 
@@ -440,7 +427,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         q.yside = yside;
         const unsigned off = unsigned(rs) * unsigned(nx) + unsigned(xs);
 #pragma unroll
-        for (int v = 0; v < NV; v++) q.u[v] = LDS_STATE ? ld_plane_any(uin, P, v, off) : ld_plane(uin, P, v, off);
+        for (int v = 0; v < NV; v++) q.u[v] = ld_plane(uin, P, v, off);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
             for (int v = 0; v < NV; v++) q.l[v] = ld_plane(lin, P, v, off);

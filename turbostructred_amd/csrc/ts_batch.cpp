@@ -449,13 +449,16 @@ void retile(ts_batch* b, Family& f) {
     const long long waves = (long long)b->ncu * f.occupancy * 4 / 2;
     f.cand = tuning_candidate(b, f);
     if (std::get<0>(f.key) == 1) {
-        // 1-D: a work item is a group of G consecutive 60-cell strips (G = 8/4/2/1, the largest
-        // that still gives half of the wavefronts an item): fewer descriptor loads and dt atomics
+        // 1-D: a work item is a group of G consecutive 60-cell strips.  One strip per item
+        // unless the strips outnumber the wavefronts many times over (then G = 4 or 2, the
+        // largest that still leaves 8 items per wavefront): an item costs a descriptor read and
+        // two returned atomics for the dt, cheap next to the parallelism of small items (the
+        // autotuner picks 1 for every load of the 4096-tube workload but the full one)
         int G = 1;
-        for (int cand = 8; cand > 1; cand /= 2) {
+        for (int cand = 4; cand > 1; cand /= 2) {
             long long items = 0;
             for (const Problem* q : f.members) items += (q->desc.nstrips + cand - 1) / cand;
-            if (items >= waves) {
+            if (items >= 16 * waves) {
                 G = cand;
                 break;
             }

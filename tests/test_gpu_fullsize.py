@@ -19,13 +19,20 @@ and against the oracle at those sizes (the oracle running on the host's cores):
   lines and planes of the domain) to 1e-11.  Measured: dt within 5e-16, normwise 1.3e-15,
   elementwise 1.4e-15 for rho and E and 9.6e-13 for the momenta (that is, 1e-15 of the largest
   momentum where the floor applies).
+
+and for the batched form at its full size:
+
+* the 4096 shock tubes of the batched workload (N ~ logU[64, 4096], exact Godunov, FP32
+  residual, t_end in-domain) all resident in one batch: a spread of the tubes, the smallest and
+  the largest among them, end bitwise equal (state, time and step count) to the same tube solved
+  alone, so sharing the persistent launches with 4095 others changes nothing.
 """
 import os
 
 import numpy as np
 import pytest
 
-from turbostructred_amd import capi
+from turbostructred_amd import capi, workloads
 from oracle import pyoracle
 import util
 
@@ -170,3 +177,32 @@ def test_c3_256_matches_oracle():
 
     a, b, dg, dr = against_oracle(cfg, util.fill_state(cfg, f), steps)
     check_full(a, b, dg, dr, elem_tol=1e-11)
+
+
+def test_c4_4096_tube_batch_equals_single_solves():
+    tubes = workloads.shock_tubes(4096, seed=2024, precision=capi.FP32_RESIDUAL)
+    b = capi.Batch(hbm_budget_bytes=4 << 30)
+    tickets = [b.submit(cfg, U, 10**6, te) for cfg, U, te in tubes]
+    passes = 0
+    while b.run(32) > 0:
+        passes += 1
+        assert passes < 5000, "the batch did not drain"
+    sizes = np.array([cfg.nx for cfg, _, _ in tubes])
+    pick = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))} | set(range(0, 4096, 512)))
+    got = {}
+    for i, t in enumerate(tickets):
+        done, st, info = b.poll(t)
+        assert done
+        if i in pick:
+            got[i] = (st, info)
+    b.close()
+    for i in pick:
+        cfg, U, te = tubes[i]
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        ainfo = s.step(10**6, te, te)
+        alone = s.get_state()
+        s.close()
+        st, info = got[i]
+        assert info.steps_done == ainfo.steps_done and info.time == ainfo.time, (i, cfg.nx)
+        np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, alone))

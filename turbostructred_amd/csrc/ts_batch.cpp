@@ -473,16 +473,26 @@ void retile(ts_batch* b, Family& f) {
         }
         return;
     }
+    // 2-D: the tile height that minimises rounds x (rows + 2 warm-up rows), as for a single
+    // solve: the persistent wavefronts take the family's tiles in rounds of all of them, and a
+    // tile's time goes as the rows it marches.  (Taking the tallest height that still gave half
+    // the wavefronts a tile ran a lone 2048^2 solve at 32 rows, 2240 tiles for 3072 wavefronts:
+    // 1.76x the single-solve step.)
+    const long long slots = 2 * waves;
+    int hi = 4;
+    for (const Problem* q : f.members) hi = std::max(hi, std::min(q->n[1], 64));
     int ty = 4;
-    for (int cand = 32; cand > 4; cand /= 2) {
+    double best = 1e300;
+    for (int cand = 4; cand <= hi; cand++) {
         long long tiles = 0;
         for (const Problem* q : f.members) {
             const Problem& p = *q;
             tiles += (long long)p.desc.nstrips * ((p.n[1] + cand - 1) / cand);
         }
-        if (tiles >= waves) {
+        const double cost = double((tiles + slots - 1) / slots) * (cand + 2);
+        if (cost < best) {
+            best = cost;
             ty = cand;
-            break;
         }
     }
     if (f.cand >= 0) ty = kCand2D[f.cand];

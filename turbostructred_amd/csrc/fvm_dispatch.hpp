@@ -7,9 +7,9 @@
 namespace tsd {
 
 using LaunchFn = void (*)(const KArgs&, dim3, dim3, hipStream_t);
-// batched (persistent) launch: descriptors, (problem, tile) table, device tile count, number of
-// 256-thread blocks
-using BatchLaunchFn = void (*)(const KArgs*, const int2*, const int*, int, hipStream_t);
+// batched (persistent) launch: descriptors, (problem, tile) table, device tile count, per-entry
+// minima of V/SR (written by stage 1), number of 256-thread blocks
+using BatchLaunchFn = void (*)(const KArgs*, const int2*, const int*, double*, int, hipStream_t);
 using OccupancyFn = int (*)();  // resident workgroups per CU
 
 // dt pass, Euler stage (double out), SSP stage 1 (stage-array out), SSP stages 2 and 3, and the

@@ -58,9 +58,9 @@ static int single_occupancy() {
 
 #if TS_DIM != 3
 template <int RC, int LM, int MODE, typename R, typename TIn, typename TOut>
-static void launch_batched(const KArgs* d, const int2* t, const int* ntiles, int nblk, hipStream_t s) {
+static void launch_batched(const KArgs* d, const int2* t, const int* ntiles, double* part, int nblk, hipStream_t s) {
     hipLaunchKernelGGL((fvm_stage_batched<TS_DIM, RC, LM, TS_FLUX, MODE, R, TIn, TOut>), dim3(nblk), dim3(256), 0, s,
-                       d, t, ntiles);
+                       d, t, ntiles, part);
 }
 template <int RC, int LM, typename R, typename S>
 static int batched_occupancy() {

@@ -194,6 +194,47 @@ __device__ __forceinline__ void st_plane_stream(T* base, size_t P, int v, unsign
     __builtin_nontemporal_store(x, reinterpret_cast<T*>(b + cell * unsigned(sizeof(T))));
 }
 
+// The batched kernel reads its descriptors, and so every state and geometry pointer, from memory:
+// as generic pointers their accesses became flat instructions (which also wait on the scalar
+// loads' counter) and the wave-uniform row geometry came from vector loads, whose waits also
+// waited for the stencil prefetch.  Its instances (GP) address the state planes and per-lane
+// geometry as global memory and the row geometry, read-only for the whole run, as constant
+// memory (scalar loads).  The single-solve kernels take their pointers from kernel arguments,
+// where the compiler infers as much, and keep the generic forms (the explicit casts slowed the
+// viscous stages).
+#define TS_AS_GLOBAL __attribute__((address_space(1)))
+#define TS_AS_CONST __attribute__((address_space(4)))
+template <bool GP, typename T>
+__device__ __forceinline__ T ld_state(const T* base, size_t P, int v, unsigned cell) {
+    if constexpr (GP) {
+        const TS_AS_GLOBAL char* b = (const TS_AS_GLOBAL char*)(base + (size_t)v * P);
+        return *(const TS_AS_GLOBAL T*)(b + cell * unsigned(sizeof(T)));
+    } else {
+        return ld_plane(base, P, v, cell);
+    }
+}
+template <bool GP, typename T>
+__device__ __forceinline__ void st_state_stream(T* base, size_t P, int v, unsigned cell, T x) {
+    if constexpr (GP) {
+        TS_AS_GLOBAL char* b = (TS_AS_GLOBAL char*)(base + (size_t)v * P);
+        __builtin_nontemporal_store(x, (TS_AS_GLOBAL T*)(b + cell * unsigned(sizeof(T))));
+    } else {
+        st_plane_stream(base, P, v, cell, x);
+    }
+}
+// per-lane geometry
+template <bool GP, typename T>
+__device__ __forceinline__ T ld_geo(const T* p, int i) {
+    if constexpr (GP) return ((const TS_AS_GLOBAL T*)p)[i];
+    else return p[i];
+}
+// wave-uniform geometry (row sizes and slope factors)
+template <bool GP, typename T>
+__device__ __forceinline__ T ld_geo_uniform(const T* p, int i) {
+    if constexpr (GP) return ((const TS_AS_CONST T*)p)[i];
+    else return p[i];
+}
+
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -231,16 +272,6 @@ __device__ inline void dt_apply(StepCtl* ctl, double cfl, double mn) {
     }
 }
 
-// dt_apply on the minimum that the contributors of a batched launch posted with atomics, each
-// having waited for its own atomic minimum to be performed (wave_reduce_finalize): the minimum
-// is read with an atomic at the same coherence point, and no fence is needed.  Re-arms the
-// reduction cells.
-__device__ inline void dt_finalize(StepCtl* ctl, double cfl) {
-    dt_apply(ctl, cfl, bitsd(atomicAdd(&ctl->min_bits, 0ull)));
-    atomicExch(&ctl->min_bits, ~0ull);
-    atomicExch(&ctl->blocks_done, 0u);
-}
-
 // Single solves: the step's dt from the `n` per-tile minima of V/SR that a dt pass or fused
 // stage 1 stored, launched right after it.  Plain stores and one workgroup's reduction instead of
 // one same-address atomic per wavefront: those serialise where the wavefronts finish together,
@@ -263,20 +294,29 @@ __global__ void __launch_bounds__(1024) dt_finalize_kernel(StepCtl* ctl, double 
     }
 }
 
-// Wavefront min of V/SR, one atomic min and one counter increment per wavefront with no
-// barrier; the last of `count` contributors finalizes dt (the batched kernel, whose wavefronts
-// take tiles of different problems).
-__device__ inline void wave_reduce_finalize(StepCtl* ctl, double cfl, double m, int lane, int count) {
+// Batched solves: the dt of every resident solve from its work items' minima of V/SR, which
+// fused stage 1 stored by table entry (a solve's entries are contiguous: `ranges` holds the
+// descriptor slot, first entry and count of each member).  One wavefront per member, launched
+// right after stage 1; skipped for a solve whose run has ended.  Per-item atomics on the solve's
+// control block serialised on a lone large solve (2 x 3000 same-address atomics per stage 1 at
+// 2048^2), and their ordering needed either a fence, which writes back the L2, or waiting for a
+// returned value.
+template <int = 0>
+__global__ void __launch_bounds__(256) dt_finalize_batched(const KArgs* __restrict__ descs, const int4* __restrict__ ranges,
+                                                           const int* __restrict__ nmem_p, const double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int nm = __builtin_amdgcn_readfirstlane(*nmem_p);
+    const int stride = gridDim.x * (blockDim.x >> 6);
+    for (int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); i < nm; i += stride) {
+        const int4 r = ranges[i];  // slot, first entry, entries
+        const KArgs& a = descs[r.x];
+        StepCtl* ctl = a.ctl;
+        if (!ctl->active) continue;
+        double m = __builtin_inf();
+        for (int j = lane; j < r.z; j += 64) m = __builtin_fmin(m, part[r.y + j]);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
-    if (lane == 0) {
-        // Both are device-scope atomics, performed at one coherence point.  The wavefront waits
-        // for the minimum's returned value before it issues the increment, so the contributor
-        // that counts last finds every minimum already performed.  (A fence in its place wrote
-        // back the L2 for every work item: an agent-scope release on this part.)
-        const unsigned long long prev = atomicMin(&ctl->min_bits, dbits(m));
-        __asm__ volatile("" ::"v"(prev));
-        if (atomicAdd(&ctl->blocks_done, 1u) == unsigned(count - 1)) dt_finalize(ctl, cfl);
+        for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+        if (lane == 0) dt_apply(ctl, a.g.cfl, m);
     }
 }
 
@@ -364,7 +404,7 @@ __device__ inline void add_sources(const VisArgs& v, const R (&u)[NV], double vo
 // LDS_STATE instances read and write the state through `uin_arg` / `uout_arg` with plane stride
 // `plane_arg` (LDS arrays, addressed generically) with plain stores.
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut, int VIS = 0,
-          bool DT_ARG = false, bool LDS_STATE = false>
+          bool DT_ARG = false, bool LDS_STATE = false, bool GP = false>
 __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, double& block_min,
                                            double* rn = nullptr, double dt_arg = 0.0, const void* uin_arg = nullptr,
                                            void* uout_arg = nullptr, size_t plane_arg = 0) {
@@ -388,17 +428,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     const double dt = MT::sr ? 0.0 : (DT_ARG ? dt_arg : ctl->dt);
 
     CellGeo<DIM, R> geo;
-    const double hx = g.h[0][xs + 1];
-    const double ihx = g.ih[0][xs + 1];
+    const double hx = ld_geo<GP>(g.h[0], xs + 1);
+    const double ihx = ld_geo<GP>(g.ih[0], xs + 1);
     geo.hh[0] = R(0.5 * hx);
-    geo.rc[0] = R(g.rc[0][xs]);
-    geo.rl[0] = R(g.rl[0][xs]);
-    geo.rr[0] = R(g.rr[0][xs]);
+    geo.rc[0] = R(ld_geo<GP>(g.rc[0], xs));
+    geo.rl[0] = R(ld_geo<GP>(g.rl[0], xs));
+    geo.rr[0] = R(ld_geo<GP>(g.rr[0], xs));
     geo.eps2 = R(g.eps2);
     constexpr bool VENKAT = RECON == RC_LINLIM && LIM == LIM_VENKAT;
     // per-cell eps^2 on non-uniform grids (a wave-uniform branch, Venkatakrishnan instances only)
     const bool ev_cells = VENKAT && g.ev[0] != nullptr;
-    const double ev_x = ev_cells ? g.ev[0][xs] : 0.0;
+    const double ev_x = ev_cells ? ld_geo<GP>(g.ev[0], xs) : 0.0;
     if (ev_cells && DIM == 1) geo.eps2 = R(ev_x);
 
     const TIn* __restrict__ uin = LDS_STATE ? (const TIn*)uin_arg : (const TIn*)a.uin;
@@ -407,7 +447,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     const size_t P = LDS_STATE ? plane_arg : a.plane;
     auto store = [&](int v, unsigned off, TOut x) {
         if constexpr (LDS_STATE) st_plane(uo, P, v, off, x);
-        else st_plane_stream(uo, P, v, off, x);
+        else st_state_stream<GP>(uo, P, v, off, x);
     };
 
     // One stencil value, the input state at a (possibly ghost) cell of row r, in two steps: the
@@ -427,10 +467,10 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         q.yside = yside;
         const unsigned off = unsigned(rs) * unsigned(nx) + unsigned(xs);
 #pragma unroll
-        for (int v = 0; v < NV; v++) q.u[v] = ld_plane(uin, P, v, off);
+        for (int v = 0; v < NV; v++) q.u[v] = ld_state<GP>(uin, P, v, off);
         if constexpr (MODE == MODE_F2) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) q.l[v] = ld_plane(lin, P, v, off);
+            for (int v = 0; v < NV; v++) q.l[v] = ld_state<GP>(lin, P, v, off);
         }
     };
     auto complete_row = [&](const RawRow& q, R (&u)[NV]) {
@@ -473,7 +513,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
     auto load_inc = [&](unsigned off, IncT (&d)[NV]) {
         if (inc) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) d[v] = ld_plane(incsrc, P, v, off);
+            for (int v = 0; v < NV; v++) d[v] = ld_state<GP>(incsrc, P, v, off);
         }
     };
 
@@ -494,7 +534,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 if (inc && MODE == MODE_F2) {
                     const double d = 0.25 * dt * (double(dc[v]) + double(res[v]) * ivol);
                     if (out_lane) {
-                        st_plane_stream((TOut*)a.dout, P, v, off, TOut(d));
+                        st_state_stream<GP>((TOut*)a.dout, P, v, off, TOut(d));
                         store(v, off, TOut(un[v] + d));
                     }
                 } else if (inc && MODE == MODE_S3) {
@@ -517,7 +557,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
         load_row(0, c);
         double un[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) un[v] = MT::use_un ? ld_plane(a.un, P, v, unsigned(xs)) : 0.0;
+        for (int v = 0; v < NV; v++) un[v] = MT::use_un ? ld_state<GP>(a.un, P, v, unsigned(xs)) : 0.0;
         IncT dc[NV];
         load_inc(unsigned(xs), dc);
 #pragma unroll
@@ -560,7 +600,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 sr = sr + viscous_sr(rl, rr, g.gamma, a.v.mu, double(Sx), hx);
             }
             if (out_lane) block_min = dt_candidate(hx, sr);
-            if (ghost_x_lane) block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1], double(Sx * lam)));
+            if (ghost_x_lane) block_min = __builtin_fmin(block_min, dt_candidate(ld_geo_uniform<GP>(g.h[0], nx + 1), double(Sx * lam)));
         }
         if constexpr (MT::flux) {
             R res[NV];
@@ -627,18 +667,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                 if (t >= 2) {
                     const unsigned offn = unsigned(rc - 1) * unsigned(nx) + unsigned(xs);
 #pragma unroll
-                    for (int v = 0; v < NV; v++) Nc[v] = ld_plane(a.un, P, v, offn);
+                    for (int v = 0; v < NV; v++) Nc[v] = ld_state<GP>(a.un, P, v, offn);
                     load_inc(offn, Dc);
                 }
             }
             int rcs, rcside;
             map_index(rc, ny, g.periodic[1], rcs, rcside);
-            const double hy = g.h[1][rcs + 1];
+            const double hy = ld_geo_uniform<GP>(g.h[1], rcs + 1);
             geo.hh[1] = R(0.5 * hy);
-            geo.rc[1] = R(g.rc[1][rcs]);
-            geo.rl[1] = R(g.rl[1][rcs]);
-            geo.rr[1] = R(g.rr[1][rcs]);
-            if (ev_cells) geo.eps2 = R(ev_x * g.ev[1][rcs]);
+            geo.rc[1] = R(ld_geo_uniform<GP>(g.rc[1], rcs));
+            geo.rl[1] = R(ld_geo_uniform<GP>(g.rl[1], rcs));
+            geo.rr[1] = R(ld_geo_uniform<GP>(g.rr[1], rcs));
+            if (ev_cells) geo.eps2 = R(ev_x * ld_geo_uniform<GP>(g.ev[1], rcs));
             R nb[4][NV], face[4][NV];
             {
                 R U0[NV];
@@ -681,24 +721,24 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                         }
                         if constexpr (VIS)
                             sr = sr + viscous_sr(double(GL[0]), double(GR[0]), g.gamma, a.v.mu, double(Sy), hx * g.h[1][rc]);
-                        if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][rc], sr));
+                        if (out_lane) block_min = __builtin_fmin(block_min, dt_candidate(hx * ld_geo_uniform<GP>(g.h[1], rc), sr));
                     }
                     if (rc == ny && out_lane)  // top ghost row: only this face (ERK:1305)
-                        block_min = __builtin_fmin(block_min, dt_candidate(hx * g.h[1][ny + 1], double(Sy * lam)));
+                        block_min = __builtin_fmin(block_min, dt_candidate(hx * ld_geo_uniform<GP>(g.h[1], ny + 1), double(Sy * lam)));
                     lam_gb = lam;
                 }
                 if constexpr (MT::flux) {
                     if (t >= 2) {
                         // finalize row rc-1 = the row in slot sm
                         const int rf = rc - 1;
-                        const double ivol = ihx * g.ih[1][rf + 1];
+                        const double ivol = ihx * ld_geo_uniform<GP>(g.ih[1], rf + 1);
                         R res[NV], Um[NV];
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
                             res[v] = cy[v][lane] - G[v] * Sy;
                             Um[v] = ring[wl][sm][v][lane];
                         }
-                        finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, Dc, ivol, hx * g.h[1][rf + 1]);
+                        finish(unsigned(rf) * unsigned(nx) + unsigned(xs), Um, res, Nc, Dc, ivol, hx * ld_geo_uniform<GP>(g.h[1], rf + 1));
                     }
 #pragma unroll
                     for (int v = 0; v < NV; v++) cy[v][lane] = G[v] * Sy;
@@ -754,7 +794,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, int tile, int lane, d
                             srp = srp + double(Sy * lam_gb);
                         }
                         if (ghost_x_lane)  // right ghost column (ERK:1225)
-                            block_min = __builtin_fmin(block_min, dt_candidate(g.h[0][nx + 1] * hy, double(Sx * lamx)));
+                            block_min = __builtin_fmin(block_min, dt_candidate(ld_geo_uniform<GP>(g.h[0], nx + 1) * hy, double(Sx * lamx)));
                     }
                 }
             }
@@ -917,14 +957,16 @@ __global__ void __launch_bounds__(64 * wg1d_waves<FLUX>) fvm_euler1d_workgroup(K
 // the persistent grid takes tiles w, w + W, w + 2W, ... of the table (W = wavefronts in the grid):
 // no claim atomics, which serialise at one address at tens of ns each, and every wavefront's
 // loop ends after at most ceil(ntiles / W) tiles.  Neighbouring wavefront ids sit on one XCD, so
-// neighbouring tiles of a problem share an L2.  The dt of each problem comes from a wavefront-level
-// minimum and the last tile of that problem.  A cell's arithmetic depends only on its own problem,
-// so a result is the same alone or in a mix.
+// neighbouring tiles of a problem share an L2.  Stage 1 stores each table entry's wavefront-level
+// minimum of V/SR in `part` (by entry), and dt_finalize_batched turns each problem's entries into
+// its dt.  A cell's arithmetic depends only on its own problem, and the minimum is exact, so a
+// result is the same alone or in a mix.
 
 template <int DIM, int RECON, int LIM, int FLUX, int MODE, typename R, typename TIn, typename TOut>
 __global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict__ descs,
                                                          const int2* __restrict__ table,
-                                                         const int* __restrict__ ntiles_p) {
+                                                         const int* __restrict__ ntiles_p,
+                                                         double* __restrict__ part) {
     using MT = ModeTraits<MODE>;
     const int lane = threadIdx.x & 63;
     // loop control in SGPRs: every quantity below is uniform over the wavefront
@@ -945,21 +987,24 @@ __global__ void __launch_bounds__(256) fvm_stage_batched(const KArgs* __restrict
             if (!ctl->go) continue;
         }
         double m = __builtin_inf();
-        int nwork = a.ntiles;
         if constexpr (DIM == 1) {
             // a 1-D work item is a group of a.ty consecutive strips, walked by one wavefront
             const int s0 = e.y * a.ty;
             const int s1 = (s0 + a.ty < a.ntiles) ? s0 + a.ty : a.ntiles;
             for (int sidx = s0; sidx < s1; sidx++) {
                 double ms = __builtin_inf();
-                stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, sidx, lane, ms);
+                stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, 0, false, false, true>(a, sidx, lane, ms);
                 m = __builtin_fmin(m, ms);
             }
-            nwork = (a.ntiles + a.ty - 1) / a.ty;
         } else {
-            stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut>(a, e.y, lane, m);
+            stage_tile<DIM, RECON, LIM, FLUX, MODE, R, TIn, TOut, 0, false, false, true>(a, e.y, lane, m);
         }
-        if constexpr (MT::sr) wave_reduce_finalize(ctl, a.g.cfl, m, lane, nwork);
+        if constexpr (MT::sr) {
+            // the entry's minimum; dt_finalize_batched reduces each solve's entries after the launch
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) m = __builtin_fmin(m, __shfl_xor(m, o));
+            if (lane == 0) part[t] = m;
+        }
     }
 }
 

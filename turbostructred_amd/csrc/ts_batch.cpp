@@ -160,6 +160,13 @@ struct Family {
     int2* d_table = nullptr;
     int* d_ntiles = nullptr;  // tiles in the table (read by the kernels)
     size_t cap_desc = 0, cap_table = 0;
+    // stage 1's minimum of V/SR per table entry, and per member its slot, first entry and entry
+    // count (the members' entries are contiguous in the table), read by dt_finalize_batched
+    double* d_part = nullptr;
+    int4* d_ranges = nullptr;
+    int* d_nmem = nullptr;
+    size_t cap_ranges = 0;
+    int h_nmem = 0;
     // resident problems in descriptor order.  A Problem is owned by ts_batch::probs through a
     // unique_ptr, so its address is stable: the run thread reads members without the map (and
     // without the lock), while submit inserts into and poll erases from the map under the lock.
@@ -519,13 +526,24 @@ int rebuild_tables(ts_batch* b, Family& f) {
     }
     if (nt > f.cap_table) {
         if (f.d_table) BCHK(hipFree(f.d_table));
+        if (f.d_part) BCHK(hipFree(f.d_part));
         f.d_table = nullptr;
+        f.d_part = nullptr;
         f.cap_table = std::max<size_t>(nt * 2, 4096);
         BCHK(hipMalloc(&f.d_table, f.cap_table * sizeof(int2)));
+        BCHK(hipMalloc(&f.d_part, f.cap_table * sizeof(double)));
+        realloc = true;
+    }
+    if (f.members.size() > f.cap_ranges) {
+        if (f.d_ranges) BCHK(hipFree(f.d_ranges));
+        f.d_ranges = nullptr;
+        f.cap_ranges = std::max<size_t>(f.members.size() * 2, 256);
+        BCHK(hipMalloc(&f.d_ranges, f.cap_ranges * sizeof(int4)));
         realloc = true;
     }
     if (!f.d_ntiles) {
         BCHK(hipMalloc(&f.d_ntiles, sizeof(int)));
+        BCHK(hipMalloc(&f.d_nmem, sizeof(int)));
         realloc = true;
     }
     if (realloc && f.graph) {
@@ -565,12 +583,20 @@ int rebuild_tables(ts_batch* b, Family& f) {
         BCHK(hipGetLastError());
     }
     std::vector<int2> table;
+    std::vector<int4> ranges;
     table.reserve(nt);
-    for (const Problem* q : f.members)
+    ranges.reserve(f.members.size());
+    for (const Problem* q : f.members) {
+        ranges.push_back(make_int4(q->slot, (int)table.size(), q->ntiles, 0));
         for (int t = 0; t < q->ntiles; t++) table.push_back(make_int2(q->slot, t));
+    }
     if (nt) BCHK(hipMemcpyAsync(f.d_table, table.data(), nt * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+    if (!ranges.empty())
+        BCHK(hipMemcpyAsync(f.d_ranges, ranges.data(), ranges.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream));
     f.h_ntiles = (int)nt;
+    f.h_nmem = (int)ranges.size();
     BCHK(hipMemcpyAsync(f.d_ntiles, &f.h_ntiles, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipMemcpyAsync(f.d_nmem, &f.h_nmem, sizeof(int), hipMemcpyHostToDevice, b->stream));
     BCHK(hipStreamSynchronize(b->stream));  // the copy sources are local vectors
     f.dirty = false;
     return TS_OK;
@@ -583,9 +609,12 @@ namespace {
 // launches of one batched step for family f
 void issue_step(ts_batch* b, Family& f) {
     int nblk = b->ncu * f.occupancy;
-    f.ks.bf1(f.d_desc, f.d_table, f.d_ntiles, nblk, b->stream);
-    f.ks.bf2(f.d_desc + f.cap_desc, f.d_table, f.d_ntiles, nblk, b->stream);
-    f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_ntiles, nblk, b->stream);
+    f.ks.bf1(f.d_desc, f.d_table, f.d_ntiles, f.d_part, nblk, b->stream);
+    // each member's dt from its entries' minima: one wavefront per member, 8 per CU
+    hipLaunchKernelGGL(tsd::dt_finalize_batched<0>, dim3(2 * b->ncu), dim3(256), 0, b->stream, f.d_desc, f.d_ranges,
+                       f.d_nmem, f.d_part);
+    f.ks.bf2(f.d_desc + f.cap_desc, f.d_table, f.d_ntiles, f.d_part, nblk, b->stream);
+    f.ks.bs3(f.d_desc + 2 * f.cap_desc, f.d_table, f.d_ntiles, f.d_part, nblk, b->stream);
 }
 
 // `timed`: events bracket the launches themselves (a graph is built before the first event)
@@ -714,6 +743,9 @@ int ts_batch_destroy(ts_batch* b) {
         if (f.d_desc) (void)hipFree(f.d_desc);
         if (f.d_table) (void)hipFree(f.d_table);
         if (f.d_ntiles) (void)hipFree(f.d_ntiles);
+        if (f.d_part) (void)hipFree(f.d_part);
+        if (f.d_ranges) (void)hipFree(f.d_ranges);
+        if (f.d_nmem) (void)hipFree(f.d_nmem);
         if (f.d_fresh) (void)hipFree(f.d_fresh);
         if (f.d_fresh_idx) (void)hipFree(f.d_fresh_idx);
         for (hipEvent_t e : f.ev)

@@ -129,6 +129,10 @@ struct Problem {
     int slot = -1;       // index of its descriptors in its family's descriptor blocks
     StepCtl final_ctl{};  // its control block when it finished (published with its state)
     bool uploaded = false;  // its descriptors are in place for the family's current tiling
+    // admission: the host sources of its asynchronous uploads, kept until the copy stream is
+    // synchronised
+    std::vector<double> geo_host;
+    StepCtl init_ctl{};
 };
 
 // Tile-shape autotuning of a family (ts_batch_config.autotune_tiles): for each load bucket
@@ -309,6 +313,7 @@ int family_of(ts_batch* b, const ts_config& c) {
 // the device staging buffer holds at least `doubles` values (only the pass thread uses it)
 int ensure_stage(ts_batch* b, size_t doubles) {
     if (doubles <= b->stage_cap) return TS_OK;
+    BCHK(hipStreamSynchronize(b->cstream));  // earlier uploads may still read the old buffer
     if (b->d_stage) BCHK(hipFree(b->d_stage));
     b->d_stage = nullptr;
     b->stage_cap = 0;
@@ -317,7 +322,10 @@ int ensure_stage(ts_batch* b, size_t doubles) {
     return TS_OK;
 }
 
-// upload geometry/state/control of a problem into its arena block and fill its descriptor
+// upload geometry/state/control of a problem into its arena block and fill its descriptor.  The
+// copies and the layout conversion go on the copy stream, so that they overlap the kernels of
+// a pass in flight; the caller synchronises that stream before the problem runs and before the
+// host sources (init, geo_host, init_ctl) are released.
 int admit(ts_batch* b, Problem& p) {
     const ts_config& c = p.cfg;
     Layout L = layout_of(c);
@@ -330,7 +338,8 @@ int admit(ts_batch* b, Problem& p) {
     g.gamma = c.gamma;
     g.eps = c.eps;
     g.cfl = c.cfl;
-    std::vector<double> geo;
+    std::vector<double>& geo = p.geo_host;
+    geo.clear();
     geo.reserve(L.geo_doubles);
     double* dgeo = (double*)(base + L.geo);
     bool uniform = true;
@@ -378,24 +387,22 @@ int admit(ts_batch* b, Problem& p) {
         g.bc[m].rho = c.bc[m].density;
     }
     for (int s = 0; s < 6; s++) g.side_marker[s] = c.side_marker[s];
-    BCHK(hipMemcpyAsync(dgeo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, b->stream));
-    // state: the AoS array with ghosts crosses as it is and is converted on the device
+    BCHK(hipMemcpyAsync(dgeo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, b->cstream));
+    // state: the AoS array with ghosts crosses as it is and is converted on the device (the
+    // staging buffer is reused by the next admission in the stream's order)
     size_t n = p.ncell;
     int r = ensure_stage(b, p.init.size());
     if (r) return r;
-    BCHK(hipMemcpyAsync(b->d_stage, p.init.data(), p.init.size() * 8, hipMemcpyHostToDevice, b->stream));
-    BCHK(tsr::device_aos_to_soa(b->d_stage, (double*)(base + L.u0), c.ndims, p.n, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));  // the staging buffer is reused
-    p.init.clear();
-    p.init.shrink_to_fit();
-    StepCtl ctl{};
+    BCHK(hipMemcpyAsync(b->d_stage, p.init.data(), p.init.size() * 8, hipMemcpyHostToDevice, b->cstream));
+    BCHK(tsr::device_aos_to_soa(b->d_stage, (double*)(base + L.u0), c.ndims, p.n, b->cstream));
+    StepCtl& ctl = p.init_ctl;
+    ctl = StepCtl{};
     ctl.t_max = p.t_end > 0 ? p.t_end : 1e300;
     ctl.next_snap = p.t_end > 0 ? p.t_end : 0.0;
     ctl.max_steps = p.max_steps;
     ctl.active = p.max_steps > 0 ? 1 : 0;
     ctl.min_bits = ~0ull;
-    // synchronous: the source lives on this stack frame
-    BCHK(hipMemcpy(base + L.ctl, &ctl, sizeof(StepCtl), hipMemcpyHostToDevice));
+    BCHK(hipMemcpyAsync(base + L.ctl, &ctl, sizeof(StepCtl), hipMemcpyHostToDevice, b->cstream));
     KArgs& d = p.desc;
     d.uin = nullptr;
     d.lin = nullptr;
@@ -846,12 +853,10 @@ static int complete_retiring(ts_batch* b) {
     return TS_OK;
 }
 
-// one admission + advance pass; the caller is either the server thread or, when no server
-// runs, the user through ts_batch_run.  Runs under run_mu.
-static int batch_run(ts_batch* b, int n_steps, int* pending) {
-    BCHK(hipSetDevice(b->bc.device));
-    // (1) under the lock: take queued solves off the queue in arrival order while they fit the
-    // budget, reserving their arena blocks (head-of-line waits for memory)
+// Admission: take queued solves off the queue in arrival order while they fit the budget,
+// upload them (copy stream) and attach them to their families.
+static int admit_queued(ts_batch* b) {
+    // (1) under the lock: reserve their arena blocks (head-of-line waits for memory)
     std::vector<Problem*> incoming;
     {
         std::lock_guard<std::mutex> lk(b->mu);
@@ -867,34 +872,61 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
             incoming.push_back(&p);
         }
     }
+    if (incoming.empty()) return TS_OK;
     // (2) without the lock: upload their geometry, state and control blocks, so that submit and
     // poll on other threads do not wait behind the copies
     std::vector<int> admitted(incoming.size(), 0);
     for (size_t i = 0; i < incoming.size(); i++) admitted[i] = admit(b, *incoming[i]) == TS_OK;
+    BCHK(hipStreamSynchronize(b->cstream));
+    for (Problem* p : incoming) {
+        std::vector<double>().swap(p->init);
+        std::vector<double>().swap(p->geo_host);
+    }
     // (3) under the lock again: attach them to their families (or fail them)
+    std::lock_guard<std::mutex> lk(b->mu);
+    for (size_t i = 0; i < incoming.size(); i++) {
+        Problem& p = *incoming[i];
+        if (!admitted[i]) {
+            b->arena.release(p.off, p.bytes);
+            p.status = 2;
+            p.error = 1;
+            continue;
+        }
+        Family& fam = b->families[p.family];
+        if (fam.free_slots.empty()) {
+            p.slot = fam.slot_end++;
+        } else {
+            p.slot = fam.free_slots.back();
+            fam.free_slots.pop_back();
+        }
+        p.uploaded = false;
+        fam.members.push_back(&p);
+        b->families[p.family].dirty = true;
+    }
+    return TS_OK;
+}
+
+// one admission + advance pass; the caller is either the server thread or, when no server
+// runs, the user through ts_batch_run.  Runs under run_mu.  While solves are resident, the
+// queued ones are uploaded after this pass's kernels are issued (the copies overlap them) and
+// take their first steps in the next pass; an idle batch admits them before its kernels.
+static int batch_run(ts_batch* b, int n_steps, int* pending) {
+    BCHK(hipSetDevice(b->bc.device));
+    bool busy;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        size_t nres = 0;
+        for (Family& f : b->families) nres += f.members.size();
+        busy = nres > 0 && n_steps > 0;
+    }
+    if (!busy) {
+        int r = admit_queued(b);
+        if (r) return r;
+    }
     std::vector<Problem*> resident;
     size_t nfam = 0;  // families that exist now; later ones are appended without moving these
     {
         std::lock_guard<std::mutex> lk(b->mu);
-        for (size_t i = 0; i < incoming.size(); i++) {
-            Problem& p = *incoming[i];
-            if (!admitted[i]) {
-                b->arena.release(p.off, p.bytes);
-                p.status = 2;
-                p.error = 1;
-                continue;
-            }
-            Family& fam = b->families[p.family];
-            if (fam.free_slots.empty()) {
-                p.slot = fam.slot_end++;
-            } else {
-                p.slot = fam.free_slots.back();
-                fam.free_slots.pop_back();
-            }
-            p.uploaded = false;
-            fam.members.push_back(&p);
-            b->families[p.family].dirty = true;
-        }
         nfam = b->families.size();
     }
     for (size_t fi = 0; fi < nfam; fi++)
@@ -915,10 +947,12 @@ static int batch_run(ts_batch* b, int n_steps, int* pending) {
                 f.pass_cell_steps = 0;
             }
         }
-        // the previous pass's finished solves, copied out while these kernels run
+        // the previous pass's finished solves, copied out while these kernels run (their arena
+        // blocks are free after this), then the queued solves' uploads, also under the kernels
         {
             int r = complete_retiring(b);
             if (r) return r;
+            if (busy && (r = admit_queued(b))) return r;
         }
         // control blocks of every resident solve back to the host
         if (b->h_ctl_cap < resident.size()) {

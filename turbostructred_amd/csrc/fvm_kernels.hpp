@@ -52,8 +52,6 @@ struct StepCtl {
     int go;      // 1 if the current step runs (set by the dt pass)
     int error;   // 1 = dt not finite and positive
     int pad0;
-    unsigned long long min_bits;  // running min of V/SR as ordered bits
-    unsigned int blocks_done, pad1;
     double* dt_log;
     long long dt_log_base, dt_log_cap;  // dt of iteration `it` goes to dt_log[it - base]
     double rnorm[5];                    // residual norms of the last step (MODE_RN)
@@ -235,8 +233,6 @@ __device__ __forceinline__ T ld_geo_uniform(const T* p, int i) {
     else return p[i];
 }
 
-__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
-__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 // V/SR of one cell; a NaN or negative spectral radius (non-physical state) becomes 0 so that
 // the step fails with dt = 0 instead of silently skipping the cell in the minimum.

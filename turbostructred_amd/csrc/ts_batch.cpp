@@ -403,7 +403,6 @@ int admit(ts_batch* b, Problem& p) {
     ctl.next_snap = p.t_end > 0 ? p.t_end : 0.0;
     ctl.max_steps = p.max_steps;
     ctl.active = p.max_steps > 0 ? 1 : 0;
-    ctl.min_bits = ~0ull;
     BCHK(hipMemcpyAsync(base + L.ctl, &ctl, sizeof(StepCtl), hipMemcpyHostToDevice, b->cstream));
     KArgs& d = p.desc;
     d.uin = nullptr;

@@ -529,8 +529,6 @@ int run_steps(ts_ctx* c, int max_steps, double t_max, double next_snap) {
     h->active = max_steps > 0 ? 1 : 0;
     h->go = 0;
     h->error = 0;
-    h->min_bits = ~0ull;
-    h->blocks_done = 0;
     h->dt_log = c->dt_log;
     h->dt_log_base = c->dt_log_base;
     h->dt_log_cap = c->dt_log ? c->dt_log_cap : 0;

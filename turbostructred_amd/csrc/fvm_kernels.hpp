@@ -12,8 +12,8 @@
 //
 // Modes
 //   MODE_DT  spectral radius of every cell from the face eigenvalues, min(V/SR) over the grid
-//            (wave shuffle, one atomic min per wavefront); a one-wavefront kernel launched
-//            after it turns the minimum into dt (ERK:1427-1437).  Used before forward Euler.
+//            (wave shuffle, one stored minimum per tile); a one-workgroup kernel launched
+//            after it turns the minima into dt (ERK:1427-1437).  Used before forward Euler.
 //   MODE_S1  U1 = U + dt L(U)          forward Euler (kernel.h:319-331) / SSP stage 1
 //   MODE_S2  U2 = 3/4 Un + 1/4 (U1 + dt L(U1))
 //   MODE_S3  U  = 1/3 Un + 2/3 (U2 + dt L(U2))   (written in place over Un: Un is read

@@ -438,10 +438,7 @@ KArgs stage_desc(const Problem& p, int kernel, char* arena) {
 }
 
 // Tiling of a family, chosen again whenever its membership changes (results do not depend on
-// the tiling).  2-D: the tallest of 32/16/8/4 rows that still gives a tile to at least half
-// of the persistent grid's wavefronts.  Under load, tall tiles spend less on the two warm-up
-// rows and the per-tile dt atomics; a lightly loaded batch keeps short tiles so that each step
-// stays short.
+// the tiling): the load rule in retile, or the autotuner's candidates below.
 int next_candidate(const LoadTuning& lt) {
     if (lt.chosen >= 0) return lt.chosen;
     int done = 0;
@@ -467,7 +464,7 @@ void retile(ts_batch* b, Family& f) {
         // 1-D: a work item is a group of G consecutive 60-cell strips.  One strip per item
         // unless the strips outnumber the wavefronts many times over (then G = 4 or 2, the
         // largest that still leaves 8 items per wavefront): an item costs a descriptor read and
-        // two returned atomics for the dt, cheap next to the parallelism of small items (the
+        // a stored minimum for the dt, cheap next to the parallelism of small items (the
         // autotuner picks 1 for every load of the 4096-tube workload but the full one)
         int G = 1;
         for (int cand = 4; cand > 1; cand /= 2) {

@@ -216,7 +216,7 @@ int init_device_ctx(ts_ctx* c, const ts_config* cfg) {
         c->ty = 1;
         c->nstrips = (c->n[0] + STRIP - 1) / STRIP;
         c->ntiles = c->nstrips;
-        // four strips per block: the dt reduction costs two same-address atomics per block
+        // four strips (one per wavefront) per block
         c->block_dim = dim3(256);
         c->grid_dim = dim3((c->ntiles + 3) / 4);
     } else if (c->nd == 2) {

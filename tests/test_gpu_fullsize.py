@@ -20,6 +20,10 @@ and against the oracle at those sizes (the oracle running on the host's cores):
   elementwise 1.4e-15 for rho and E and 9.6e-13 for the momenta (that is, 1e-15 of the largest
   momentum where the floor applies).
 
+and the per-tile dt reduction at its largest tile count: a 2^22-cell 1-D grid (69906 strips),
+forward Euler and SSP-RK3, against the oracle over 6 steps (measured: dt within 3e-16,
+normwise 2e-15);
+
 and for the batched form at its full size:
 
 * the 4096 shock tubes of the batched workload (N ~ logU[64, 4096], exact Godunov, FP32
@@ -174,6 +178,27 @@ def test_c3_256_matches_oracle():
         v = -np.cos(x) * np.sin(y) * np.cos(z)
         p = 1 / (g * 0.01) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16
         return util.prim2cons(1 + 0 * x, u, v, 0 * x, p, g)
+
+    a, b, dg, dr = against_oracle(cfg, util.fill_state(cfg, f), steps)
+    check_full(a, b, dg, dr, elem_tol=1e-11)
+
+
+@pytest.mark.parametrize("rk", [1, 3])
+def test_1d_4m_cells_dt_reduction_matches_oracle(rk):
+    """A 1-D grid of 2^22 cells (69906 strips, so 69906 per-tile minima of V/SR per dt pass or
+    fused stage 1, reduced by one workgroup), with a smooth profile whose fastest cell is unique,
+    limited linear BJ + HLLC, forward Euler (separate dt pass) or SSP-RK3 (fused): dt of every
+    step and the state against the oracle."""
+    n, steps = 1 << 22, 6
+    cfg = capi.make_config(ndims=1, nx=n, cfl=0.45, riemann=capi.RIEMANN_HLLC, recon=capi.RECON_LINEAR_LIM,
+                           limiter=capi.LIMITER_BJ, rk_stages=rk, bcs=[dict(type=capi.BC_NATURAL)])
+    g = cfg.gamma
+
+    def f(x, y, z):
+        rho = 1.0 + 0.5 * np.sin(2 * np.pi * 3 * x)
+        u = 0.3 * np.exp(-((x - 0.6173) / 0.05) ** 2)
+        p = 1.0 + 0.25 * np.cos(2 * np.pi * 5 * x)
+        return util.prim2cons(rho, u, 0 * x, 0 * x, p, g)
 
     a, b, dg, dr = against_oracle(cfg, util.fill_state(cfg, f), steps)
     check_full(a, b, dg, dr, elem_tol=1e-11)

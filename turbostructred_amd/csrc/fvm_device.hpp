@@ -451,8 +451,9 @@ __device__ inline void hllc_flux(R gamma, const R (&UL)[NV], const R (&UR)[NV], 
 //         (remainder below 5e-18), scaled by 2^k.
 // On 2e6 bases in [1e-12, 1e4] with the solver's exponents this is within 2e-15 of the correctly
 // rounded power for y <= 1 and 1.5e-14 for y up to 7 (the rounding of t = y ln x), the same as
-// exp(y ln x) with the library's ln and exp, at about a third of their instructions.  One
-// out-of-line copy: inlined at each call site the kernels outgrew the instruction cache.
+// exp(y ln x) with the library's ln and exp, in 101 VALU instructions against their 139 (and 223
+// for the library's correctly rounded pow).  One out-of-line copy: inlined at each call site the
+// kernels outgrew the instruction cache.
 // x = 0 gives 0; x < 0 or NaN gives NaN; x = +inf gives +inf.
 static __device__ __noinline__ double dpow(double x, double y) {
     if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");

@@ -526,7 +526,8 @@ __device__ inline SideDev side_dev(double cA, double cB, double c2, double r, do
 
 // f_K(p) and its derivative (GodunovSolverPerfectGasEOS.h:58-97; derivative Toro eq. 4.37);
 // z = (g-1)/2g.  The reciprocals and the square root are the refined estimates (a few ulps).
-__device__ inline double fk_dev(const SideDev& s, double z, double p, double& d, double& pz) {
+// `pzk` = (p/pK)^z, used on the rarefaction side.
+__device__ inline double fk_dev(const SideDev& s, double p, double pzk, double& d, double& pz) {
     if (p > s.pk) {
         const double ipb = frcp(p + s.B);
         const double q = fsqrt(s.A * ipb);
@@ -534,12 +535,22 @@ __device__ inline double fk_dev(const SideDev& s, double z, double p, double& d,
         pz = 0.0;
         return (p - s.pk) * q;
     }
-    // rarefaction: one power, (p/pK)^z, gives f, f' = (p/pK)^(z-1)/(r aK) and the tail speed of
+    // rarefaction: the power (p/pK)^z gives f, f' = (p/pK)^(z-1)/(r aK) and the tail speed of
     // the fan (the sampler reuses pz)
     const double pr = p * s.ipk;
-    pz = dpow(pr, z);
+    pz = pzk;
     d = pz * frcp(pr) * s.ira;
     return (pz - 1.0) * s.c2a;
+}
+// (p/pL)^z and (p/pR)^z with one power: (p/pR)^z = (p/pL)^z (pL/pR)^z, the last factor formed
+// once per face; skipped when neither side is a rarefaction at p
+__device__ inline void side_powers(const SideDev& l, const SideDev& r, double z, double rho_lr, double p, double& pzl,
+                                   double& pzr) {
+    pzl = pzr = 0.0;
+    if (!(p > l.pk) || !(p > r.pk)) {
+        pzl = dpow(p * l.ipk, z);
+        pzr = pzl * rho_lr;
+    }
 }
 
 // Star pressure and velocity, wave speeds and smax; the star densities are formed by the sampler,
@@ -562,6 +573,7 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
     const double z = (g - 1.0) / (2.0 * g);
     const double cA = 2.0 / (g + 1.0), cB = (g - 1.0) / (g + 1.0), c2 = 2.0 / (g - 1.0);
     const SideDev sl = side_dev(cA, cB, c2, rl, pl, al), sr = side_dev(cA, cB, c2, rr, pr, ar);
+    const double rho_lr = dpow(pl * sr.ipk, z);  // (pL/pR)^z
     double p = 0.0;
     // Inputs that are not a physical state (zero, negative or NaN density or pressure) give a
     // NaN star pressure straight away.  Such inputs reach the solver on halo lanes, which are
@@ -591,7 +603,9 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         if (!(p > 0.0)) p = 1e-12 * pmin;
         for (int it = 1; it <= 60; it++) {
             double dl, dr, zl, zr;
-            double fl = fk_dev(sl, z, p, dl, zl), fr = fk_dev(sr, z, p, dr, zr);
+            double ql, qr;
+            side_powers(sl, sr, z, rho_lr, p, ql, qr);
+            double fl = fk_dev(sl, p, ql, dl, zl), fr = fk_dev(sr, p, qr, dr, zr);
             pe = p, fle = fl, fre = fr, zle = zl, zre = zr;
             double f = fl + fr + du;
             double pn = p - fdiv(f, dl + dr);
@@ -608,8 +622,10 @@ __device__ inline StarDev star_state_dev(double g, double rl, double ul, double 
         f1 = fle, f2 = fre, pzl = zle, pzr = zre;
     } else {
         double d1, d2;
-        f1 = fk_dev(sl, z, p, d1, pzl);
-        f2 = fk_dev(sr, z, p, d2, pzr);
+        double ql, qr;
+        side_powers(sl, sr, z, rho_lr, p, ql, qr);
+        f1 = fk_dev(sl, p, ql, d1, pzl);
+        f2 = fk_dev(sr, p, qr, d2, pzr);
     }
     s.u = 0.5 * (ul + ur) + 0.5 * (f2 - f1);
     s.smax = 0.0;

@@ -29,7 +29,10 @@ and for the batched form at its full size:
 * the 4096 shock tubes of the batched workload (N ~ logU[64, 4096], exact Godunov, FP32
   residual, t_end in-domain) all resident in one batch: a spread of the tubes, the smallest and
   the largest among them, end bitwise equal (state, time and step count) to the same tube solved
-  alone, so sharing the persistent launches with 4095 others changes nothing.
+  alone, so sharing the persistent launches with 4095 others changes nothing;
+* the served form on the serving workload's requests (2-D four-quadrant problems, side
+  ~ logU[64, 1024], 50-500 steps, FP32 residual): 40 requests submitted to the server thread
+  while it runs end bitwise equal to each request solved alone with ts_step.
 """
 import os
 
@@ -230,4 +233,48 @@ def test_c4_4096_tube_batch_equals_single_solves():
         s.close()
         st, info = got[i]
         assert info.steps_done == ainfo.steps_done and info.time == ainfo.time, (i, cfg.nx)
+        np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, alone))
+
+
+def test_c5_served_requests_equal_single_solves():
+    import time
+    reqs = workloads.riemann_2d(40, seed=7, precision=capi.FP32_RESIDUAL)
+    b = capi.Batch(hbm_budget_bytes=8 << 30)
+    b.start_server()
+    try:
+        tickets = []
+        for cfg, U, steps in reqs:
+            tickets.append(b.submit(cfg, U, steps))
+            time.sleep(0.001)
+        got = {}
+        deadline = time.time() + 120
+        while len(got) < len(tickets) and time.time() < deadline:
+            for t in tickets:
+                if t not in got:
+                    try:
+                        d, st, info = b.poll(t)
+                    except capi.TSError:  # a request whose state turns unphysical
+                        got[t] = None
+                        continue
+                    if d:
+                        got[t] = (st, info)
+            time.sleep(0.002)
+        assert len(got) == len(tickets), "the server did not finish the requests"
+    finally:
+        b.stop_server()
+        b.close()
+    assert sum(v is not None for v in got.values()) >= 30
+    for (cfg, U, steps), t in zip(reqs, tickets):
+        s = capi.Solver(cfg)
+        s.set_state(U)
+        if got[t] is None:  # fails alone as well
+            with pytest.raises(capi.TSError):
+                s.step(steps)
+            s.close()
+            continue
+        info = s.step(steps)
+        alone = s.get_state()
+        s.close()
+        st, binfo = got[t]
+        assert binfo.steps_done == info.steps_done and binfo.time == info.time, (cfg.nx, steps)
         np.testing.assert_array_equal(capi.interior(cfg, st), capi.interior(cfg, alone))
